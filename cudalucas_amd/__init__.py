@@ -1,0 +1,304 @@
+"""MI355X-native Lucas-Lehmer engine: Python view of the C-ABI.
+
+The product is ``libcudalucas_amd.so`` (HIP kernels for gfx950 behind the
+C-ABI in ``include/ll_api.h``) and the CUDALucas-compatible driver binary
+``CUDALucasAMD``.  This module binds the C-ABI with ctypes for tests,
+``bench.py`` and Python callers.  There is no fallback: if the library is
+missing or cannot reach a GPU, calls raise.
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcudalucas_amd.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "ll_api.h")
+
+LL_OK = 0
+LL_ERR_ARG = -1
+LL_ERR_UNSUPPORTED = -2
+LL_ERR_HIP = -3
+LL_ERR_STATE = -4
+LL_ERR_ALLOC = -5
+LL_ERR_ROUNDOFF = -6
+
+LL_OPT_FORCE_SWEEPS = 1
+LL_OPT_NO_GRAPH = 2
+
+_lib = None
+
+
+class LLError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("LL error %d: %s" % (code, msg))
+        self.code = code
+
+
+class _Options(ctypes.Structure):
+    _fields_ = [
+        ("device", ctypes.c_int32),
+        ("threads", ctypes.c_int32 * 2),
+        ("width", ctypes.c_uint32),
+        ("height", ctypes.c_uint32),
+        ("lines", ctypes.c_uint32),
+        ("flags", ctypes.c_uint32),
+    ]
+
+
+class _Info(ctypes.Structure):
+    _fields_ = [
+        ("p", ctypes.c_uint32),
+        ("n", ctypes.c_uint32),
+        ("path", ctypes.c_uint32),
+        ("width", ctypes.c_uint32),
+        ("height", ctypes.c_uint32),
+        ("lines", ctypes.c_uint32),
+        ("stream", ctypes.c_void_p),
+        ("iterations", ctypes.c_uint64),
+    ]
+
+
+def build():
+    """Compile the HIP library and the driver (make, in-tree)."""
+    import subprocess
+
+    subprocess.run(["make", "-s", "-j8", "-C", os.path.join(_HERE, "csrc")], check=True)
+
+
+def exported_symbols():
+    """Function names declared in include/ll_api.h."""
+    with open(HEADER_PATH) as f:
+        text = f.read()
+    return sorted(set(re.findall(r"\b(ll_[a-z0-9_]+)\s*\(", text)))
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libcudalucas_amd.so is not built (run __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    u32, u64, i32, f32 = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32, ctypes.c_float
+    P = ctypes.c_void_p
+    L.ll_create.argtypes = [u32, u32, ctypes.POINTER(_Options), ctypes.POINTER(P)]
+    L.ll_destroy.argtypes = [P]
+    L.ll_destroy.restype = None
+    L.ll_load_digits.argtypes = [P, P, u32]
+    L.ll_iterate.argtypes = [P, u32, P, ctypes.POINTER(f32)]
+    L.ll_read_digits.argtypes = [P, P, ctypes.POINTER(u32)]
+    L.ll_reset_error.argtypes = [P]
+    L.ll_error_max.argtypes = [P]
+    L.ll_error_max.restype = f32
+    L.ll_get_info.argtypes = [P, ctypes.POINTER(_Info)]
+    L.ll_last_error.argtypes = [P]
+    L.ll_last_error.restype = ctypes.c_char_p
+    L.ll_debug_inject.argtypes = [P, u32, u64, f32]
+    L.ll_digit_width.argtypes = [u32, u32, u32]
+    L.ll_unpack_bits.argtypes = [u32, u32, P, P]
+    L.ll_unpack_bits.restype = None
+    L.ll_balance_digits.argtypes = [u32, u32, P]
+    L.ll_balance_digits.restype = None
+    L.ll_standardize_digits.argtypes = [u32, u32, P]
+    L.ll_pack_bits.argtypes = [u32, u32, P, P]
+    L.ll_pack_bits.restype = None
+    L.ll_residue_words.argtypes = [u32, u32, P, u32, P]
+    L.ll_res64.argtypes = [u32, u32, P, u32]
+    L.ll_res64.restype = u64
+    L.ll_initial_digits.argtypes = [u32, u32, u32, P]
+    L.ll_initial_digits.restype = None
+    L.ll_checkpoint_crc.argtypes = [P, ctypes.c_size_t]
+    L.ll_checkpoint_crc.restype = u32
+    L.ll_fft_length_count.restype = ctypes.c_int
+    L.ll_fft_length_at.argtypes = [ctypes.c_int]
+    L.ll_fft_supported.argtypes = [u32]
+    L.ll_choose_fft_length.argtypes = [u32, ctypes.POINTER(ctypes.c_int)]
+    L.ll_fft_bounds.argtypes = [u32, ctypes.POINTER(i32), ctypes.POINTER(i32)]
+    L.ll_fft_bounds.restype = None
+    L.ll_fft_from_str.argtypes = [ctypes.c_char_p]
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def nwords(p):
+    return (p + 31) // 32
+
+
+# ------------------------------------------------------------------ codec
+
+def digit_width(p, n, j):
+    return lib().ll_digit_width(p, n, j)
+
+
+def unpack_bits(p, n, packed):
+    packed = np.ascontiguousarray(packed, dtype=np.uint32)
+    d = np.zeros(n, dtype=np.int32)
+    lib().ll_unpack_bits(p, n, _ptr(packed), _ptr(d))
+    return d
+
+
+def balance_digits(p, n, digits):
+    d = np.array(digits, dtype=np.int32)
+    lib().ll_balance_digits(p, n, _ptr(d))
+    return d
+
+
+def standardize_digits(p, n, digits):
+    d = np.array(digits, dtype=np.int32)
+    z = lib().ll_standardize_digits(p, n, _ptr(d))
+    return d, bool(z)
+
+
+def pack_bits(p, n, digits):
+    d = np.ascontiguousarray(digits, dtype=np.int32)
+    out = np.zeros(nwords(p), dtype=np.uint32)
+    lib().ll_pack_bits(p, n, _ptr(d), _ptr(out))
+    return out
+
+
+def residue_words(p, n, digits, offset):
+    d = np.ascontiguousarray(digits, dtype=np.int32)
+    out = np.zeros(nwords(p) + 2, dtype=np.uint32)
+    rc = lib().ll_residue_words(p, n, _ptr(d), offset, _ptr(out))
+    if rc:
+        raise LLError(rc, "ll_residue_words")
+    return out[: nwords(p)]
+
+
+def res64(p, n, digits, offset):
+    d = np.ascontiguousarray(digits, dtype=np.int32)
+    return int(lib().ll_res64(p, n, _ptr(d), offset))
+
+
+def initial_digits(p, n, offset):
+    d = np.zeros(n, dtype=np.int32)
+    lib().ll_initial_digits(p, n, offset, _ptr(d))
+    return d
+
+
+def checkpoint_crc(data):
+    buf = np.frombuffer(bytes(data), dtype=np.uint8)
+    return int(lib().ll_checkpoint_crc(_ptr(buf), buf.size))
+
+
+def fft_lengths():
+    L = lib()
+    return [L.ll_fft_length_at(i) for i in range(L.ll_fft_length_count())]
+
+
+def fft_supported(n):
+    return bool(lib().ll_fft_supported(n))
+
+
+def choose_fft_length(p, index=-1):
+    idx = ctypes.c_int(index)
+    n = lib().ll_choose_fft_length(p, ctypes.byref(idx))
+    return n, idx.value
+
+
+def fft_bounds(p):
+    lb, ub = ctypes.c_int32(), ctypes.c_int32()
+    lib().ll_fft_bounds(p, ctypes.byref(lb), ctypes.byref(ub))
+    return lb.value, ub.value
+
+
+def fft_from_str(s):
+    return lib().ll_fft_from_str(s.encode())
+
+
+# ----------------------------------------------------------------- engine
+
+class LLContext:
+    """One exponent at one FFT length on one GPU (ll_create .. ll_destroy)."""
+
+    def __init__(self, p, n, device=0, width=0, height=0, lines=0, flags=0, threads=(0, 0)):
+        L = lib()
+        opts = _Options()
+        opts.device = device
+        opts.threads[0], opts.threads[1] = threads
+        opts.width, opts.height, opts.lines, opts.flags = width, height, lines, flags
+        h = ctypes.c_void_p()
+        rc = L.ll_create(p, n, ctypes.byref(opts), ctypes.byref(h))
+        self._h = h.value
+        if rc:
+            msg = L.ll_last_error(self._h).decode() if self._h else "ll_create"
+            self.close()
+            raise LLError(rc, msg)
+        self.p, self.n = p, n
+
+    def _check(self, rc):
+        if rc:
+            raise LLError(rc, lib().ll_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ll_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        self.close()
+
+    def info(self):
+        i = _Info()
+        self._check(lib().ll_get_info(self._h, ctypes.byref(i)))
+        return {
+            "p": i.p, "n": i.n, "path": "resident" if i.path == 0 else "sweeps",
+            "width": i.width, "height": i.height, "lines": i.lines,
+            "stream": i.stream, "iterations": i.iterations,
+        }
+
+    def load_digits(self, digits, offset):
+        d = np.ascontiguousarray(digits, dtype=np.int32)
+        assert d.size == self.n
+        self._check(lib().ll_load_digits(self._h, _ptr(d), offset))
+
+    def load_initial(self, offset=0):
+        self.load_digits(initial_digits(self.p, self.n, offset), offset)
+
+    def iterate(self, count, per_iter=False):
+        errs = np.zeros(count, dtype=np.float32) if per_iter else None
+        m = ctypes.c_float()
+        self._check(lib().ll_iterate(self._h, count, None if errs is None else _ptr(errs), ctypes.byref(m)))
+        return (m.value, errs) if per_iter else m.value
+
+    def read_digits(self):
+        d = np.zeros(self.n, dtype=np.int32)
+        off = ctypes.c_uint32()
+        self._check(lib().ll_read_digits(self._h, _ptr(d), ctypes.byref(off)))
+        return d, off.value
+
+    def residue(self):
+        """(unshifted residue words, Res64) of the current state."""
+        d, off = self.read_digits()
+        w = residue_words(self.p, self.n, d, off)
+        return w, int(w[0]) | (int(w[1]) << 32)
+
+    def error_max(self):
+        return lib().ll_error_max(self._h)
+
+    def reset_error(self):
+        self._check(lib().ll_reset_error(self._h))
+
+    def inject(self, kind, after=0, value=0.0):
+        self._check(lib().ll_debug_inject(self._h, kind, after, value))
+
+
+def run_ll(p, n, iterations, offset=0, **kw):
+    """s_iterations (s_0 = 4) at FFT length n: (residue words, Res64, max error)."""
+    with LLContext(p, n, **kw) as c:
+        c.load_initial(offset)
+        m = c.iterate(iterations) if iterations else 0.0
+        w, r = c.residue()
+    return w, r, m

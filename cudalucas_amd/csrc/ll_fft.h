@@ -1,0 +1,298 @@
+// Device-side building blocks for the Lucas-Lehmer iteration on gfx950:
+// complex helpers, radix-R DFT butterflies, and a Stockham autosort FFT that
+// runs NB independent length-L transforms held in LDS by one workgroup.
+//
+// Conventions: forward transforms use exp(-2 pi i jk/L), inverse transforms
+// exp(+2 pi i jk/L); neither is normalised.  Twiddle tables hold the forward
+// roots exp(-2 pi i e/L); inverse passes use their conjugates.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ll_types.h"
+
+namespace llamd {
+
+__device__ __forceinline__ cplx cadd(cplx a, cplx b) { return {a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ cplx csub(cplx a, cplx b) { return {a.x - b.x, a.y - b.y}; }
+__device__ __forceinline__ cplx cconj(cplx a) { return {a.x, -a.y}; }
+__device__ __forceinline__ cplx cscale(cplx a, double s) { return {a.x * s, a.y * s}; }
+__device__ __forceinline__ cplx cmul(cplx a, cplx b) {
+  return {fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x)};
+}
+// a * conj(b)
+__device__ __forceinline__ cplx cmulc(cplx a, cplx b) {
+  return {fma(a.x, b.x, a.y * b.y), fma(a.y, b.x, -a.x * b.y)};
+}
+__device__ __forceinline__ cplx csqr(cplx a) {
+  return {(a.x + a.y) * (a.x - a.y), 2.0 * a.x * a.y};
+}
+// multiply by -i (forward) or +i (inverse): the primitive 4th root used by
+// the transform direction.
+template <bool INV>
+__device__ __forceinline__ cplx mul_w4(cplx a) {
+  return INV ? cplx{-a.y, a.x} : cplx{a.y, -a.x};
+}
+
+// ---------------------------------------------------------------- radices
+constexpr double kC8 = 0.7071067811865476;   // cos(pi/4)
+constexpr double kC16 = 0.9238795325112867;  // cos(pi/8)
+constexpr double kS16 = 0.3826834323650898;  // sin(pi/8)
+constexpr double kS3 = 0.8660254037844386;   // sin(2pi/3)
+constexpr double kC51 = 0.30901699437494745, kS51 = 0.9510565162951535;
+constexpr double kC52 = -0.8090169943749475, kS52 = 0.5877852522924731;
+constexpr double kC71 = 0.6234898018587335, kS71 = 0.7818314824680298;
+constexpr double kC72 = -0.2225209339563144, kS72 = 0.9749279121818236;
+constexpr double kC73 = -0.9009688679024191, kS73 = 0.4338837391175581;
+
+template <bool INV>
+__device__ __forceinline__ void dft2(cplx &a, cplx &b) {
+  cplx t = a;
+  a = cadd(t, b);
+  b = csub(t, b);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft4(cplx &a, cplx &b, cplx &c, cplx &d) {
+  cplx s0 = cadd(a, c), d0 = csub(a, c);
+  cplx s1 = cadd(b, d), d1 = mul_w4<INV>(csub(b, d));
+  a = cadd(s0, s1);
+  c = csub(s0, s1);
+  b = cadd(d0, d1);
+  d = csub(d0, d1);
+}
+
+// multiply by w8^k, w8 = exp(-+2 pi i/8)
+template <bool INV, int K>
+__device__ __forceinline__ cplx mul_w8(cplx a) {
+  if constexpr (K == 0) return a;
+  else if constexpr (K == 2) return mul_w4<INV>(a);
+  else if constexpr (K == 1) {
+    // (1 -+ i)/sqrt2
+    return INV ? cplx{kC8 * (a.x - a.y), kC8 * (a.x + a.y)}
+               : cplx{kC8 * (a.x + a.y), kC8 * (a.y - a.x)};
+  } else {  // K == 3: (-1 -+ i)/sqrt2
+    return INV ? cplx{-kC8 * (a.x + a.y), kC8 * (a.x - a.y)}
+               : cplx{kC8 * (a.y - a.x), -kC8 * (a.x + a.y)};
+  }
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft8(cplx *v) {
+  dft4<INV>(v[0], v[2], v[4], v[6]);
+  dft4<INV>(v[1], v[3], v[5], v[7]);
+  v[3] = mul_w8<INV, 1>(v[3]);
+  v[5] = mul_w8<INV, 2>(v[5]);
+  v[7] = mul_w8<INV, 3>(v[7]);
+  // outputs: X[k + 4m] from even part E[k] (= v[2k]) and odd O[k] (= v[2k+1])
+  cplx e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+  cplx o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+  v[0] = cadd(e0, o0); v[4] = csub(e0, o0);
+  v[1] = cadd(e1, o1); v[5] = csub(e1, o1);
+  v[2] = cadd(e2, o2); v[6] = csub(e2, o2);
+  v[3] = cadd(e3, o3); v[7] = csub(e3, o3);
+}
+
+// multiply by w16^k for k in {1,...,6, 9}, w16 = exp(-+2 pi i/16)
+template <bool INV, int K>
+__device__ __forceinline__ cplx mul_w16(cplx a) {
+  if constexpr (K % 2 == 0) {
+    return mul_w8<INV, K / 2>(a);
+  } else {
+    // cos and sin of K*pi/8 for the odd K used by dft16
+    constexpr double c = K == 1 ? kC16 : K == 3 ? kS16 : -kC16;  // K == 9
+    constexpr double s = K == 1 ? kS16 : K == 3 ? kC16 : -kS16;
+    static_assert(K == 1 || K == 3 || K == 9, "unused twiddle");
+    // forward w = c - i s, inverse w = c + i s
+    constexpr double wi = INV ? s : -s;
+    return {fma(a.x, c, -a.y * wi), fma(a.x, wi, a.y * c)};
+  }
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft16(cplx *v) {
+  // 4 x 4: columns over stride-4 inputs, twiddle w16^(i*u), then rows.
+  dft4<INV>(v[0], v[4], v[8], v[12]);
+  dft4<INV>(v[1], v[5], v[9], v[13]);
+  dft4<INV>(v[2], v[6], v[10], v[14]);
+  dft4<INV>(v[3], v[7], v[11], v[15]);
+  v[5] = mul_w16<INV, 1>(v[5]);
+  v[9] = mul_w16<INV, 2>(v[9]);
+  v[13] = mul_w16<INV, 3>(v[13]);
+  v[6] = mul_w16<INV, 2>(v[6]);
+  v[10] = mul_w16<INV, 4>(v[10]);
+  v[14] = mul_w16<INV, 6>(v[14]);
+  v[7] = mul_w16<INV, 3>(v[7]);
+  v[11] = mul_w16<INV, 6>(v[11]);
+  v[15] = mul_w16<INV, 9>(v[15]);
+  dft4<INV>(v[0], v[1], v[2], v[3]);
+  dft4<INV>(v[4], v[5], v[6], v[7]);
+  dft4<INV>(v[8], v[9], v[10], v[11]);
+  dft4<INV>(v[12], v[13], v[14], v[15]);
+  // element (u, i) = v[4u + i] holds X[u + 4 i]; transpose to natural order
+  cplx t[16];
+#pragma unroll
+  for (int u = 0; u < 4; u++)
+#pragma unroll
+    for (int i = 0; i < 4; i++) t[u + 4 * i] = v[4 * u + i];
+#pragma unroll
+  for (int k = 0; k < 16; k++) v[k] = t[k];
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft3(cplx *v) {
+  cplx s = cadd(v[1], v[2]);
+  cplx d = csub(v[1], v[2]);
+  cplx m = {fma(-0.5, s.x, v[0].x), fma(-0.5, s.y, v[0].y)};
+  // sigma * i * sin(2pi/3) * d, sigma = -1 forward, +1 inverse
+  cplx r = INV ? cplx{-kS3 * d.y, kS3 * d.x} : cplx{kS3 * d.y, -kS3 * d.x};
+  v[0] = cadd(v[0], s);
+  v[1] = cadd(m, r);
+  v[2] = csub(m, r);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft5(cplx *v) {
+  cplx s1 = cadd(v[1], v[4]), d1 = csub(v[1], v[4]);
+  cplx s2 = cadd(v[2], v[3]), d2 = csub(v[2], v[3]);
+  cplx a1 = {v[0].x + kC51 * s1.x + kC52 * s2.x, v[0].y + kC51 * s1.y + kC52 * s2.y};
+  cplx a2 = {v[0].x + kC52 * s1.x + kC51 * s2.x, v[0].y + kC52 * s1.y + kC51 * s2.y};
+  // b_u = sum_i sin(2 pi i u / 5) d_i
+  cplx b1 = {kS51 * d1.x + kS52 * d2.x, kS51 * d1.y + kS52 * d2.y};
+  cplx b2 = {kS52 * d1.x - kS51 * d2.x, kS52 * d1.y - kS51 * d2.y};
+  // y_u = a_u + sigma i b_u ; y_{5-u} = a_u - sigma i b_u
+  cplx ib1 = INV ? cplx{-b1.y, b1.x} : cplx{b1.y, -b1.x};
+  cplx ib2 = INV ? cplx{-b2.y, b2.x} : cplx{b2.y, -b2.x};
+  v[0] = cadd(v[0], cadd(s1, s2));
+  v[1] = cadd(a1, ib1);
+  v[4] = csub(a1, ib1);
+  v[2] = cadd(a2, ib2);
+  v[3] = csub(a2, ib2);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft7(cplx *v) {
+  cplx s1 = cadd(v[1], v[6]), d1 = csub(v[1], v[6]);
+  cplx s2 = cadd(v[2], v[5]), d2 = csub(v[2], v[5]);
+  cplx s3 = cadd(v[3], v[4]), d3 = csub(v[3], v[4]);
+  // cos(2 pi i u / 7) for (u, i): u=1: c1 c2 c3 ; u=2: c2 c3 c1 ; u=3: c3 c1 c2
+  cplx a1 = {v[0].x + kC71 * s1.x + kC72 * s2.x + kC73 * s3.x,
+             v[0].y + kC71 * s1.y + kC72 * s2.y + kC73 * s3.y};
+  cplx a2 = {v[0].x + kC72 * s1.x + kC73 * s2.x + kC71 * s3.x,
+             v[0].y + kC72 * s1.y + kC73 * s2.y + kC71 * s3.y};
+  cplx a3 = {v[0].x + kC73 * s1.x + kC71 * s2.x + kC72 * s3.x,
+             v[0].y + kC73 * s1.y + kC71 * s2.y + kC72 * s3.y};
+  // sin(2 pi i u / 7): u=1: s1 s2 s3 ; u=2: s2 -s3 -s1 ; u=3: s3 -s1 s2
+  cplx b1 = {kS71 * d1.x + kS72 * d2.x + kS73 * d3.x, kS71 * d1.y + kS72 * d2.y + kS73 * d3.y};
+  cplx b2 = {kS72 * d1.x - kS73 * d2.x - kS71 * d3.x, kS72 * d1.y - kS73 * d2.y - kS71 * d3.y};
+  cplx b3 = {kS73 * d1.x - kS71 * d2.x + kS72 * d3.x, kS73 * d1.y - kS71 * d2.y + kS72 * d3.y};
+  cplx ib1 = INV ? cplx{-b1.y, b1.x} : cplx{b1.y, -b1.x};
+  cplx ib2 = INV ? cplx{-b2.y, b2.x} : cplx{b2.y, -b2.x};
+  cplx ib3 = INV ? cplx{-b3.y, b3.x} : cplx{b3.y, -b3.x};
+  v[0] = cadd(v[0], cadd(s1, cadd(s2, s3)));
+  v[1] = cadd(a1, ib1);
+  v[6] = csub(a1, ib1);
+  v[2] = cadd(a2, ib2);
+  v[5] = csub(a2, ib2);
+  v[3] = cadd(a3, ib3);
+  v[4] = csub(a3, ib3);
+}
+
+template <int R, bool INV>
+__device__ __forceinline__ void dft(cplx *v) {
+  if constexpr (R == 2) dft2<INV>(v[0], v[1]);
+  else if constexpr (R == 3) dft3<INV>(v);
+  else if constexpr (R == 4) dft4<INV>(v[0], v[1], v[2], v[3]);
+  else if constexpr (R == 5) dft5<INV>(v);
+  else if constexpr (R == 7) dft7<INV>(v);
+  else if constexpr (R == 8) dft8<INV>(v);
+  else if constexpr (R == 16) dft16<INV>(v);
+  else static_assert(R == 2, "unsupported radix");
+}
+
+// ------------------------------------------------------------ radix plans
+constexpr int odd_part(int L) { while (L % 2 == 0) L /= 2; return L; }
+constexpr int log2_pow2(int L) { int k = 0; while ((1 << k) < L) k++; return k; }
+
+// Radix of pass `pass` for a length-L transform: odd prime factors first
+// (7, 5, 3), then the power-of-two part split into 16s with an 8, 4 or
+// 8 x 8 remainder chosen to avoid radix 2.
+constexpr int plan_radix(int L, int pass) {
+  int rads[32] = {0};
+  int n = 0;
+  int o = odd_part(L);
+  while (o % 7 == 0) { rads[n++] = 7; o /= 7; }
+  while (o % 5 == 0) { rads[n++] = 5; o /= 5; }
+  while (o % 3 == 0) { rads[n++] = 3; o /= 3; }
+  int k = log2_pow2(L / odd_part(L));
+  // k = 4a + r
+  int r = k % 4;
+  int a = k / 4;
+  if (r == 1 && a >= 1) { a -= 1; rads[n++] = 8; rads[n++] = 4; }      // 16 = 8*2 -> 8*4*... : 2^5 = 8*4
+  else if (r == 2) { rads[n++] = 4; }
+  else if (r == 3) { rads[n++] = 8; }
+  else if (r == 1) { rads[n++] = 2; }
+  for (int i = 0; i < a; i++) rads[n++] = 16;
+  return pass < n ? rads[pass] : 0;
+}
+
+// LDS index with one pad slot every 16 complex values.
+__device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
+__host__ __device__ constexpr int lpitch(int L) { return L + L / 16 + 1; }
+
+// One Stockham pass over NB transforms of length L held at buf[b*LP + lpad(i)].
+// Pass with radix R and span NS reads x[j + i*L/R], multiplies by
+// w^(i*(j mod NS)) with w = exp(-+2 pi i/(NS*R)), applies a length-R DFT and
+// writes to (j/NS)*NS*R + (j mod NS) + u*NS.
+template <int L, int R, int NS, int NB, int T, bool INV>
+__device__ __forceinline__ void stockham_pass(cplx *buf, const cplx *__restrict__ tw) {
+  constexpr int Q = L / R;
+  constexpr int NBF = NB * Q;
+  constexpr int PER = (NBF + T - 1) / T;
+  constexpr int LP = lpitch(L);
+  cplx v[PER][R];
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int s = 0; s < PER; s++) {
+    const int u = tid + s * T;
+    if ((NBF % T == 0) || u < NBF) {
+      const int b = u / Q, j = u % Q;
+      const int k = j % NS;
+#pragma unroll
+      for (int i = 0; i < R; i++) {
+        cplx a = buf[b * LP + lpad(j + i * Q)];
+        if (NS > 1 && i > 0) {
+          cplx w = tw[(i * k) * (L / (NS * R))];
+          a = INV ? cmulc(a, w) : cmul(a, w);
+        }
+        v[s][i] = a;
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < PER; s++) {
+    const int u = tid + s * T;
+    if ((NBF % T == 0) || u < NBF) {
+      const int b = u / Q, j = u % Q;
+      const int k = j % NS;
+      dft<R, INV>(v[s]);
+      const int base = (j / NS) * NS * R + k;
+#pragma unroll
+      for (int i = 0; i < R; i++) buf[b * LP + lpad(base + i * NS)] = v[s][i];
+    }
+  }
+  __syncthreads();
+}
+
+template <int L, int NB, int T, bool INV, int PASS = 0, int NS = 1>
+__device__ __forceinline__ void fft_lds(cplx *buf, const cplx *__restrict__ tw) {
+  if constexpr (NS < L) {
+    constexpr int R = plan_radix(L, PASS);
+    static_assert(R > 0, "bad plan");
+    stockham_pass<L, R, NS, NB, T, INV>(buf, tw);
+    fft_lds<L, NB, T, INV, PASS + 1, NS * R>(buf, tw);
+  }
+}
+
+}  // namespace llamd

@@ -1,0 +1,284 @@
+// Host-side arithmetic of the LL engine: variable-base digit codec, Res64,
+// checkpoint checksum, FFT-length policy and the extended-precision tables
+// (roots of unity, IBDWT weights) the kernels read.
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "ll_api.h"
+#include "ll_host.h"
+
+namespace llamd {
+
+// ------------------------------------------------------------------ tables
+
+// cos and sin of -2 pi e / L (the forward root), from an exact octant
+// reduction of the rational e/L and long double evaluation on [0, pi/4].
+void root_of_unity(uint64_t e, uint64_t L, double *re, double *im) {
+  e %= L;
+  const uint64_t o = (8 * e) / L;
+  const uint64_t r = 8 * e - o * L;
+  const long double quarter_pi = 0.78539816339744830961566084581987572L;
+  const long double t = quarter_pi * (long double)r / (long double)L;
+  const long double u = quarter_pi * (long double)(L - r) / (long double)L;
+  const long double ct = cosl(t), st = sinl(t), cu = cosl(u), su = sinl(u);
+  long double c, s;  // cos, sin of +2 pi e / L
+  switch (o) {
+    case 0: c = ct; s = st; break;
+    case 1: c = su; s = cu; break;
+    case 2: c = -st; s = ct; break;
+    case 3: c = -cu; s = su; break;
+    case 4: c = -ct; s = -st; break;
+    case 5: c = -su; s = -cu; break;
+    case 6: c = st; s = -ct; break;
+    default: c = cu; s = -su; break;
+  }
+  *re = (double)c;
+  *im = (double)-s;
+}
+
+// exp(-2 pi i e / L) - 1 for small angles, computed as
+// (-2 sin^2(theta/2), -sin(theta)) to keep the small real part accurate.
+void root_minus_one(uint64_t e, uint64_t L, double *re, double *im) {
+  const long double two_pi = 6.28318530717958647692528676655900577L;
+  const long double th = two_pi * (long double)e / (long double)L;
+  const long double h = sinl(th / 2);
+  *re = (double)(-2.0L * h * h);
+  *im = (double)(-sinl(th));
+}
+
+void roots_table(uint32_t L, std::vector<double> &out) {
+  out.resize(2 * (size_t)L);
+  for (uint32_t e = 0; e < L; e++) root_of_unity(e, L, &out[2 * e], &out[2 * e + 1]);
+}
+
+uint32_t weight_exponent(uint32_t p, uint32_t n, uint64_t j) {
+  const uint64_t b = p % n;
+  return (uint32_t)((n - (b * j) % n) % n);
+}
+
+void weight_pair(uint32_t f, uint32_t n, double *w, double *iw) {
+  const long double e = (long double)f / (long double)n;
+  *w = (double)exp2l(e);
+  *iw = (double)exp2l(-e);
+}
+
+// --------------------------------------------------------------- codec
+
+static uint64_t digit_start(uint32_t p, uint32_t n, uint64_t j) {
+  return ((uint64_t)p * j + n - 1) / n;
+}
+
+}  // namespace llamd
+
+using namespace llamd;
+
+extern "C" {
+
+int ll_digit_width(uint32_t p, uint32_t n, uint32_t j) {
+  const uint32_t b = p % n;
+  return (int)(p / n) + (weight_exponent(p, n, j) < b ? 1 : 0);
+}
+
+void ll_unpack_bits(uint32_t p, uint32_t n, const uint32_t *packed, int32_t *digits) {
+  const size_t nw = (p + 31) / 32;
+  uint64_t acc = 0;
+  int have = 0;
+  size_t w = 0;
+  for (uint32_t j = 0; j < n; j++) {
+    const int width = ll_digit_width(p, n, j);
+    while (have < width) {
+      acc |= (uint64_t)(w < nw ? packed[w] : 0u) << have;
+      have += 32;
+      w++;
+    }
+    digits[j] = (int32_t)(acc & ((1ull << width) - 1));
+    acc >>= width;
+    have -= width;
+  }
+}
+
+void ll_balance_digits(uint32_t p, uint32_t n, int32_t *digits) {
+  int32_t carry = 0;
+  for (uint32_t j = 0; j < n; j++) {
+    const int width = ll_digit_width(p, n, j);
+    const int32_t half = 1 << (width - 1);
+    digits[j] += carry;
+    carry = 0;
+    if (digits[j] >= half) {
+      digits[j] -= 2 * half;
+      carry = 1;
+    }
+  }
+  digits[0] += carry;
+}
+
+int ll_standardize_digits(uint32_t p, uint32_t n, int32_t *digits) {
+  std::vector<int64_t> d(digits, digits + n);
+  std::vector<uint8_t> width(n);
+  for (uint32_t j = 0; j < n; j++) width[j] = (uint8_t)ll_digit_width(p, n, j);
+  for (int lap = 0; lap < 64; lap++) {
+    int64_t carry = 0;
+    for (uint32_t j = 0; j < n; j++) {
+      const int64_t t = d[j] + carry;
+      d[j] = t & (((int64_t)1 << width[j]) - 1);
+      carry = t >> width[j];  // floor division
+    }
+    if (carry == 0) break;
+    d[0] += carry;  // 2^p == 1
+  }
+  bool all_ones = true, zero = true;
+  for (uint32_t j = 0; j < n; j++) {
+    if (d[j] != (((int64_t)1 << width[j]) - 1)) all_ones = false;
+    if (d[j] != 0) zero = false;
+  }
+  if (all_ones) {
+    std::fill(d.begin(), d.end(), 0);
+    zero = true;
+  }
+  for (uint32_t j = 0; j < n; j++) digits[j] = (int32_t)d[j];
+  return zero ? 1 : 0;
+}
+
+void ll_pack_bits(uint32_t p, uint32_t n, const int32_t *digits, uint32_t *packed) {
+  const size_t nw = (p + 31) / 32;
+  uint64_t acc = 0;
+  int have = 0;
+  size_t w = 0;
+  for (uint32_t j = 0; j < n; j++) {
+    acc |= (uint64_t)(uint32_t)digits[j] << have;
+    have += ll_digit_width(p, n, j);
+    while (have >= 32) {
+      if (w < nw) packed[w] = (uint32_t)acc;
+      w++;
+      acc >>= 32;
+      have -= 32;
+    }
+  }
+  if (w < nw) packed[w++] = (uint32_t)acc;
+  while (w < nw) packed[w++] = 0;
+}
+
+int ll_residue_words(uint32_t p, uint32_t n, const int32_t *digits, uint32_t offset,
+                     uint32_t *words) {
+  if (p < 3 || n == 0) return LL_ERR_ARG;
+  std::vector<int32_t> d(digits, digits + n);
+  ll_standardize_digits(p, n, d.data());
+  const size_t nw = (p + 31) / 32;
+  std::vector<uint32_t> v(nw + 1, 0);
+  ll_pack_bits(p, n, d.data(), v.data());
+  // rotate right by offset within p bits: out bit i = v bit (i + offset) mod p
+  offset %= p;
+  memset(words, 0, nw * 4);
+  auto bit = [&](uint32_t k) { return (v[k >> 5] >> (k & 31)) & 1u; };
+  for (uint32_t i = 0; i < p; i++) {
+    uint32_t s = i + offset;
+    if (s >= p) s -= p;
+    if (bit(s)) words[i >> 5] |= 1u << (i & 31);
+  }
+  return LL_OK;
+}
+
+uint64_t ll_res64(uint32_t p, uint32_t n, const int32_t *digits, uint32_t offset) {
+  std::vector<uint32_t> w((p + 31) / 32 + 2, 0);
+  if (ll_residue_words(p, n, digits, offset, w.data())) return 0;
+  return (uint64_t)w[0] | ((uint64_t)w[1] << 32);
+}
+
+void ll_initial_digits(uint32_t p, uint32_t n, uint32_t offset, int32_t *digits) {
+  const uint64_t bit = ((uint64_t)offset + 2) % p;
+  const uint32_t digit = (uint32_t)((bit * n) / p);
+  memset(digits, 0, sizeof(int32_t) * n);
+  digits[digit] = (int32_t)(1u << (bit - digit_start(p, n, digit)));
+}
+
+uint32_t ll_checkpoint_crc(const void *bytes, size_t len) {
+  const uint8_t *s = (const uint8_t *)bytes;
+  uint32_t crc = 0;
+  for (size_t i = 0; i < len; i++) {
+    for (int b = 7; b >= 0; b--) {
+      const uint32_t top = crc >> 31;
+      crc <<= 1;
+      if (top != ((s[i] >> b) & 1u)) crc ^= 0x04C11DB7u;
+    }
+  }
+  return crc;
+}
+
+// ------------------------------------------------------ FFT-length policy
+
+// The reference's 162 default lengths in units of 1024 words
+// (CUDALucas.cu:955-971).
+static const int k_lengths_k[] = {
+    1,     2,     4,     8,     10,    14,    16,    18,    20,    32,    36,    42,    48,
+    50,    56,    60,    64,    70,    80,    84,    96,    112,   120,   126,   128,   144,
+    160,   162,   168,   180,   192,   224,   256,   288,   320,   324,   336,   360,   384,
+    392,   400,   448,   512,   576,   640,   648,   672,   720,   768,   784,   800,   864,
+    896,   900,   1024,  1152,  1176,  1280,  1296,  1344,  1440,  1568,  1600,  1728,  1792,
+    2048,  2160,  2304,  2352,  2592,  2688,  2880,  3024,  3136,  3200,  3584,  3600,  4096,
+    4320,  4608,  4704,  5120,  5184,  5600,  5760,  6048,  6144,  6272,  6400,  6480,  7168,
+    7200,  7776,  8064,  8192,  8640,  9216,  9408,  10240, 10368, 10584, 10800, 11200, 11520,
+    12096, 12288, 12544, 12960, 13824, 14336, 14400, 16384, 17496, 18144, 19208, 19600, 20000,
+    20250, 21952, 23328, 23814, 24300, 24500, 25088, 25600, 26244, 27000, 27216, 28000, 28672,
+    31104, 31250, 32000, 32400, 32768, 33614, 34992, 36000, 36288, 38416, 39200, 39366, 40500,
+    41472, 42336, 43200, 43904, 47628, 49000, 50000, 50176, 51200, 52488, 54432, 55296, 56000,
+    57344, 60750, 62500, 64000, 64800, 65536};
+
+int ll_fft_length_count(void) { return (int)(sizeof(k_lengths_k) / sizeof(k_lengths_k[0])); }
+
+int ll_fft_length_at(int index) {
+  if (index < 0 || index >= ll_fft_length_count()) return -1;
+  return k_lengths_k[index] * 1024;
+}
+
+void ll_fft_bounds(uint32_t p, int32_t *lb, int32_t *ub) {
+  // power laws fitted by the reference (CUDALucas.cu:1034-1050)
+  const double ff1 = 0.999 * 1024.0 * 0.0000357822505975293;
+  const double ff2 = 0.99 * 1024.0 * 0.0002670641830112380;
+  const double e1 = 1.022179977969700;
+  const double e2 = 0.929905288591965;
+  if (lb) *lb = (int32_t)ceil(ff1 * exp(e1 * log((double)p)));
+  if (ub) *ub = (int32_t)floor(ff2 * exp(e2 * log((double)p)));
+}
+
+int ll_choose_fft_length(uint32_t p, int *index) {
+  const int count = ll_fft_length_count();
+  int32_t lb, ub;
+  ll_fft_bounds(p, &lb, &ub);
+  int lbi = 0;
+  while (lbi < count && ll_fft_length_at(lbi) < lb) lbi++;
+  int ubi = lbi;
+  while (ubi < count && ll_fft_length_at(ubi) <= ub) ubi++;
+  ubi--;
+  int i;
+  if (*index >= count) {  // manual length
+    i = 0;
+    while (i < count && ll_fft_length_at(i) < *index) i++;
+  } else {
+    i = *index < 0 ? lbi : *index;
+  }
+  if (i < lbi) i = lbi;
+  if (ubi >= lbi && i > ubi) i = ubi;
+  while (i < count && !ll_fft_supported((uint32_t)ll_fft_length_at(i))) i++;
+  if (i >= count) return LL_ERR_UNSUPPORTED;
+  *index = i;
+  return ll_fft_length_at(i);
+}
+
+int ll_fft_from_str(const char *s) {
+  if (!s || !*s) return -1;
+  char *end = nullptr;
+  const unsigned long v = strtoul(s, &end, 10);
+  if (end == s) return -1;
+  long mult = 1;
+  if (*end == 'k' || *end == 'K') { mult = 1024; end++; }
+  else if (*end == 'm' || *end == 'M') { mult = 1024 * 1024; end++; }
+  if (*end != '\0') return -1;
+  const unsigned long long n = (unsigned long long)v * mult;
+  if (n > 0x7fffffffull) return -1;
+  return (int)n;
+}
+
+}  // extern "C"

@@ -1,0 +1,20 @@
+// Internal host helpers shared by the engine (ll_host.cpp).
+#pragma once
+#include <stdint.h>
+
+#include <vector>
+
+namespace llamd {
+
+// exp(-2 pi i e / L), from an exact rational reduction
+void root_of_unity(uint64_t e, uint64_t L, double *re, double *im);
+// exp(-2 pi i e / L) - 1 with an accurate small real part
+void root_minus_one(uint64_t e, uint64_t L, double *re, double *im);
+// interleaved (re, im) of exp(-2 pi i e / L) for e < L
+void roots_table(uint32_t L, std::vector<double> &out);
+// f(j) = (-(p mod n) j) mod n, the IBDWT weight exponent of word j
+uint32_t weight_exponent(uint32_t p, uint32_t n, uint64_t j);
+// 2^(f/n) and 2^(-f/n)
+void weight_pair(uint32_t f, uint32_t n, double *w, double *iw);
+
+}  // namespace llamd

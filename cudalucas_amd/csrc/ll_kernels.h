@@ -1,0 +1,90 @@
+// Launch-side view of the LL kernels (ll_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "ll_types.h"
+
+namespace llamd {
+
+// exp(-2 pi i e/M) = hi[e >> shift] * (1 + lo[e & mask]), lo = (cos-1, sin)
+struct TwoLevel {
+  const cplx *hi;
+  const cplx *lo;
+  uint32_t shift;
+  uint32_t mask;
+};
+
+// Device-resident iteration state of the two-sweep path.
+struct DevState {
+  unsigned long long ticket;  // carry-sweep admission counter (monotonic)
+  uint32_t done;              // carry-sweep workgroups finished this launch
+  uint32_t offset;            // shift offset of the stored value
+};
+
+struct PlanArgs {
+  cplx *x;                 // M complex, natural order m = a*H + b
+  const cplx *twW;         // exp(-2 pi i e/W), e < W
+  const cplx *twH;         // exp(-2 pi i e/H), e < H
+  TwoLevel twM;            // exp(-2 pi i e/M), e < M
+  const double *wA, *iwA;  // 2^(+-fA[a]/n), a < W
+  const int32_t *fA;       // (-b * 2H a) mod n
+  const double *wB, *iwB;  // 2^(+-fB[i]/n), i < 2H
+  const int32_t *fB;       // (-b * i) mod n
+  uint64_t *shuttle;       // (H/R) x W epoch-tagged carries
+  DevState *st;
+  uint32_t *err;           // ring of per-iteration max errors (float bits)
+  uint32_t err_mask;
+  int32_t *digits;         // N words, natural order (load/unload)
+  double scale;            // 1 / (4 M) after the squaring round trip
+  uint32_t p, n, W, H, q, b;
+};
+
+struct ResidentDesc {
+  cplx *x;            // M complex, natural order (weighted words)
+  const double *w;    // 2^(f(j)/n)
+  const double *iw;   // 2^(-f(j)/n)
+  const cplx *tw;     // exp(-2 pi i e/M), e < M
+  float *err;         // iters floats
+  uint32_t *offset_out;
+  double scale;       // 1 / (4 M)
+  uint32_t p, m, offset, iters;
+};
+
+struct SquareKernel {
+  uint32_t h;
+  const void *fn;
+  uint32_t threads;
+};
+
+struct CarryKernel {
+  uint32_t w, r;
+  const void *carry, *load, *unload;
+  uint32_t threads;
+};
+
+struct ResidentKernel {
+  uint32_t m;
+  const void *fn;
+  uint32_t threads;
+};
+
+const SquareKernel *find_square(uint32_t H);
+const CarryKernel *find_carry(uint32_t W, uint32_t R);
+const CarryKernel *carry_table(size_t *count);
+const ResidentKernel *find_resident(uint32_t M);
+
+hipError_t launch_square(const SquareKernel *k, const PlanArgs &A, hipStream_t s);
+hipError_t launch_carry(const CarryKernel *k, const PlanArgs &A, hipStream_t s);
+hipError_t launch_load(const CarryKernel *k, const PlanArgs &A, hipStream_t s);
+hipError_t launch_unload(const CarryKernel *k, const PlanArgs &A, hipStream_t s);
+hipError_t launch_resident(const ResidentKernel *k, const ResidentDesc *d_descs, uint32_t count,
+                           hipStream_t s);
+hipError_t launch_resident_unload(const ResidentDesc *d_descs, uint32_t count, int32_t *digits,
+                                  uint32_t *err, hipStream_t s);
+hipError_t launch_resident_load(const ResidentDesc *d_descs, uint32_t count, const int32_t *digits,
+                                hipStream_t s);
+hipError_t launch_fill(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
+
+}  // namespace llamd

@@ -1,0 +1,602 @@
+// HIP kernels of the Lucas-Lehmer iteration s <- s^2 - 2 mod 2^p - 1 on gfx950.
+//
+// One iteration of the reference (CUDALucas.cu:1869-1986: cuFFT INVERSE,
+// square, cuFFT INVERSE, rcb, splice) is rebuilt in two shapes:
+//
+//  * resident_kernel: for N <= 16K words the whole M = N/2-point complex
+//    signal lives in one workgroup's LDS and the kernel loops over many
+//    iterations without touching HBM (batched: one workgroup per exponent).
+//
+//  * two sweeps over HBM for large N, M = W x H, signal in natural order
+//    (complex element m = (x[2m], x[2m+1]) at address m = a*H + b):
+//      square_sweep : per column pair (k1, W-k1): forward H-point DFT,
+//                     real-pairing squaring, inverse H-point DFT.
+//      carry_sweep  : per block of R lines b: inverse twiddle, inverse W-point
+//                     DFT, unweight/round/-2/balanced carry, carry look-back
+//                     from the previous block, reweight, forward W-point
+//                     DFT, forward twiddle.
+//    Each sweep reads and writes the signal once: 32 N bytes per iteration.
+//
+// Carries between line blocks travel through an epoch-tagged shuttle
+// (64-bit words: iteration epoch in the high half, int32 carry in the low
+// half) written and polled with agent-scope atomics, so no fence or flag is
+// needed.  Blocks take tickets from an atomic counter, so a block only waits
+// on blocks that were admitted before it.  The cyclic wrap (the last block's
+// carries feed block 0) is handled by admitting H/R + 1 workgroups per
+// launch: ticket 0 computes block 0 only to publish its carries; the last
+// ticket recomputes block 0 from the untouched input, applies the wrapped
+// carries and writes it.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ll_fft.h"
+#include "ll_kernels.h"
+
+namespace llamd {
+
+// Upper bound on carry look-back polls (a few hundred ms); reaching it marks
+// the iteration failed (error 3.0) instead of hanging the device.
+constexpr uint32_t kSpinLimit = 1u << 22;
+
+// ------------------------------------------------------------------ helpers
+
+__device__ __forceinline__ cplx twiddle_m(const TwoLevel &t, uint32_t e) {
+  const cplx h = t.hi[e >> t.shift];
+  const cplx l = t.lo[e & t.mask];  // (cos - 1, sin) of a small angle
+  return {h.x + fma(h.x, l.x, -h.y * l.y), h.y + fma(h.x, l.y, h.y * l.x)};
+}
+
+// Balanced digit of value v in base 2^width and the carry out.
+__device__ __forceinline__ int64_t balance(int64_t v, int width, int64_t &carry) {
+  const int64_t half = (int64_t)1 << (width - 1);
+  const int64_t d = ((v + half) & ((half << 1) - 1)) - half;
+  carry = (v - d) >> width;
+  return d;
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+template <int T>
+__device__ __forceinline__ float block_max(float v, float *scratch) {
+  v = wave_max(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) scratch[wave] = v;
+  __syncthreads();
+  float m = 0.f;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int w = 0; w < T / 64; w++) m = fmaxf(m, scratch[w]);
+  }
+  return m;  // valid in thread 0
+}
+
+// Position of the -2 for the iteration that maps offset -> 2*offset mod p:
+// subtract 2^(pos) from digit `digit` (CUDALucas.cu:1882-1886, exact).
+struct Minus2 {
+  uint32_t new_offset;
+  uint32_t digit;
+  int64_t value;
+};
+
+__device__ __forceinline__ Minus2 minus2_position(uint32_t offset, uint32_t p, uint32_t n) {
+  Minus2 m;
+  m.new_offset = (uint32_t)((2ull * offset) % p);
+  uint64_t bit = (uint64_t)m.new_offset + 1;
+  if (bit == p) bit = 0;
+  m.digit = (uint32_t)((bit * n) / p);
+  const uint64_t start = ((uint64_t)m.digit * p + n - 1) / n;
+  m.value = (int64_t)1 << (bit - start);
+  return m;
+}
+
+// Words of the pair (2(aH+b), 2(aH+b)+1) use f = fA[a] + fB[2b+e] (mod n).
+struct WordWeight {
+  double w, iw;
+  int width;
+};
+
+__device__ __forceinline__ WordWeight word_weight(const PlanArgs &A, int fa, double wa, double iwa,
+                                                  int i) {
+  int f = fa + A.fB[i];
+  const bool wrap = f >= (int)A.n;
+  if (wrap) f -= (int)A.n;
+  WordWeight r;
+  r.w = wa * A.wB[i] * (wrap ? 0.5 : 1.0);
+  r.iw = iwa * A.iwB[i] * (wrap ? 2.0 : 1.0);
+  r.width = (int)A.q + (f < (int)A.b ? 1 : 0);
+  return r;
+}
+
+// Real-pairing squaring of the half-length spectrum: with A = Z[k],
+// B = Z[M-k], t = exp(-2 pi i k/M), E2 = A + conj(B), O2 = -i (A - conj(B)),
+//   Z'[k]   = E2^2 + t O2^2 + 2i E2 O2
+//   Z'[M-k] = conj(E2^2 + t O2^2 - 2i E2 O2)
+// which is 4x the half-length spectrum of the squared real signal; the
+// factor 1/4 is applied with the unweighting.
+__device__ __forceinline__ void pair_square(cplx &A, cplx &B, cplx t) {
+  const cplx e2 = {A.x + B.x, A.y - B.y};
+  const cplx o2 = {A.y + B.y, B.x - A.x};
+  const cplx P = csqr(e2);
+  const cplx Q = cmul(t, csqr(o2));
+  const cplx eo = cmul(e2, o2);
+  const cplx S = {-2.0 * eo.y, 2.0 * eo.x};
+  const cplx PQ = cadd(P, Q);
+  A = cadd(PQ, S);
+  B = cconj(csub(PQ, S));
+}
+
+// ------------------------------------------------------------ square sweep
+// One workgroup per column pair (k1, W - k1), k1 in [0, W/2].  After the
+// forward H-point DFT, element k2 of column k1 holds Z[k1 + W k2]; its
+// partner M - k is (W - k1, H - 1 - k2) for k1 != 0 and (0, H - k2) for
+// k1 = 0.
+template <int H, int T>
+__global__ __launch_bounds__(T) void square_sweep(PlanArgs A) {
+  constexpr int LP = lpitch(H);
+  __shared__ cplx lds[2 * LP];
+  const int tid = threadIdx.x;
+  const uint32_t W = A.W;
+  const uint32_t k1 = blockIdx.x;
+  const uint32_t k1b = (W - k1) % W;
+  const bool pair = k1b != k1;
+  const cplx *col0 = A.x + (size_t)k1 * H;
+  const cplx *col1 = A.x + (size_t)k1b * H;
+  for (int i = tid; i < H; i += T) {
+    lds[lpad(i)] = col0[i];
+    lds[LP + lpad(i)] = col1[i];
+  }
+  __syncthreads();
+  fft_lds<H, 2, T, false>(lds, A.twH);
+  if (pair) {
+    for (int k2 = tid; k2 < H; k2 += T) {
+      const int k2b = H - 1 - k2;
+      cplx a = lds[lpad(k2)], b = lds[LP + lpad(k2b)];
+      pair_square(a, b, twiddle_m(A.twM, k1 + W * (uint32_t)k2));
+      lds[lpad(k2)] = a;
+      lds[LP + lpad(k2b)] = b;
+    }
+  } else if (k1 == 0) {
+    for (int k2 = tid; k2 <= H / 2; k2 += T) {
+      const int k2b = (H - k2) % H;
+      cplx a = lds[lpad(k2)], b = lds[lpad(k2b)];
+      pair_square(a, b, twiddle_m(A.twM, W * (uint32_t)k2));
+      lds[lpad(k2)] = a;
+      if (k2b != k2) lds[lpad(k2b)] = b;
+    }
+  } else {
+    for (int k2 = tid; k2 < (H + 1) / 2; k2 += T) {
+      const int k2b = H - 1 - k2;
+      cplx a = lds[lpad(k2)], b = lds[lpad(k2b)];
+      pair_square(a, b, twiddle_m(A.twM, k1 + W * (uint32_t)k2));
+      lds[lpad(k2)] = a;
+      if (k2b != k2) lds[lpad(k2b)] = b;
+    }
+  }
+  __syncthreads();
+  fft_lds<H, 2, T, true>(lds, A.twH);
+  cplx *out0 = A.x + (size_t)k1 * H;
+  cplx *out1 = A.x + (size_t)k1b * H;
+  for (int i = tid; i < H; i += T) {
+    out0[i] = lds[lpad(i)];
+    if (pair) out1[i] = lds[LP + lpad(i)];
+  }
+}
+
+// ------------------------------------------------------------- carry sweep
+template <int W, int R, int T>
+__global__ __launch_bounds__(T) void carry_sweep(PlanArgs A) {
+  constexpr int LP = lpitch(W);
+  constexpr int CPT = (W + T - 1) / T;  // columns per thread
+  __shared__ cplx lds[R * LP];
+  __shared__ float red[T / 64];
+  __shared__ uint64_t s_ticket;
+  __shared__ Minus2 s_m2;
+  const int tid = threadIdx.x;
+  const uint32_t H = A.H;
+  const uint32_t G = H / R;
+
+  if (tid == 0) {
+    s_ticket = atomicAdd(&A.st->ticket, 1ull);
+    s_m2 = minus2_position(A.st->offset, A.p, A.n);
+  }
+  __syncthreads();
+  const uint64_t tk = s_ticket;
+  const uint32_t epoch = (uint32_t)(tk / (G + 1));
+  const uint32_t slot = (uint32_t)(tk % (G + 1));
+  const bool redo = slot == G;
+  const bool head = slot == 0;
+  const uint32_t blk = redo ? 0 : slot;
+  const uint32_t b0 = blk * R;
+  const Minus2 m2 = s_m2;
+
+  // load R lines and undo the column twiddle
+  for (int e = tid; e < R * W; e += T) {
+    const int r = e % R, k1 = e / R;
+    cplx v = A.x[(size_t)k1 * H + b0 + r];
+    v = cmulc(v, twiddle_m(A.twM, (b0 + r) * (uint32_t)k1));
+    lds[r * LP + lpad(k1)] = v;
+  }
+  __syncthreads();
+  fft_lds<W, R, T, true>(lds, A.twW);
+
+  // round, subtract, carry along the 2R consecutive words of each column
+  float emax = 0.f;
+  int32_t dg[CPT][2 * R];
+#pragma unroll
+  for (int c = 0; c < CPT; c++) {
+    const int a = tid + c * T;
+    if (W % T == 0 || a < W) {
+      const int fa = A.fA[a];
+      const double wa = A.wA[a], iwa = A.iwA[a];
+      const uint32_t j0 = 2u * ((uint32_t)a * H + b0);
+      int64_t carry = 0;
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const cplx v = lds[r * LP + lpad(a)];
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+          const WordWeight ww = word_weight(A, fa, wa, iwa, 2 * (b0 + r) + e);
+          const double t = (e ? v.y : v.x) * (ww.iw * A.scale);
+          const double rt = rint(t);
+          emax = fmaxf(emax, (float)fabs(t - rt));
+          int64_t iv = (int64_t)rt;
+          if (j0 + 2 * r + e == m2.digit) iv -= m2.value;
+          dg[c][2 * r + e] = (int32_t)balance(iv + carry, ww.width, carry);
+        }
+      }
+      if (!redo) {
+        const int32_t c32 = (int32_t)carry;
+        if ((int64_t)c32 != carry) emax = 2.0f;  // carry overflow: flagged as a failed iteration
+        const uint64_t word = ((uint64_t)epoch << 32) | (uint32_t)c32;
+        __hip_atomic_store(&A.shuttle[(size_t)blk * W + a], word, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+  }
+
+  if (head) {
+    // block 0 is written by the redo ticket; this one only fed block 1
+    if (tid == 0) {
+      const uint32_t old = atomicAdd(&A.st->done, 1u);
+      if (old == G) {
+        A.st->done = 0;
+        A.st->offset = m2.new_offset;
+      }
+    }
+    return;
+  }
+
+  // carry-in from the previous block (for block 0: the last block, one
+  // column to the left), then reweight
+#pragma unroll
+  for (int c = 0; c < CPT; c++) {
+    const int a = tid + c * T;
+    if (W % T == 0 || a < W) {
+      const size_t src = redo ? (size_t)(G - 1) * W + (a + W - 1) % W : (size_t)(blk - 1) * W + a;
+      uint64_t word;
+      uint32_t spins = 0;
+      for (;;) {
+        word = __hip_atomic_load(&A.shuttle[src], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((uint32_t)(word >> 32) == epoch) break;
+        if (++spins > kSpinLimit) {  // never hang: flag the iteration as failed
+          emax = 3.0f;
+          word = 0;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+      const int32_t cin = (int32_t)(uint32_t)word;
+      const int fa = A.fA[a];
+      const double wa = A.wA[a], iwa = A.iwA[a];
+      {
+        const WordWeight w0 = word_weight(A, fa, wa, iwa, 2 * b0);
+        int64_t c1;
+        const int64_t d0 = balance((int64_t)dg[c][0] + cin, w0.width, c1);
+        dg[c][0] = (int32_t)d0;
+        dg[c][1] += (int32_t)c1;
+      }
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const WordWeight we = word_weight(A, fa, wa, iwa, 2 * (b0 + r));
+        const WordWeight wo = word_weight(A, fa, wa, iwa, 2 * (b0 + r) + 1);
+        lds[r * LP + lpad(a)] = {(double)dg[c][2 * r] * we.w, (double)dg[c][2 * r + 1] * wo.w};
+      }
+    }
+  }
+  __syncthreads();
+  fft_lds<W, R, T, false>(lds, A.twW);
+  for (int e = tid; e < R * W; e += T) {
+    const int r = e % R, k1 = e / R;
+    const cplx v = lds[r * LP + lpad(k1)];
+    A.x[(size_t)k1 * H + b0 + r] = cmul(v, twiddle_m(A.twM, (b0 + r) * (uint32_t)k1));
+  }
+  const float m = block_max<T>(emax, red);
+  if (tid == 0) {
+    atomicMax(&A.err[epoch & A.err_mask], __float_as_uint(m));
+    const uint32_t old = atomicAdd(&A.st->done, 1u);
+    if (old == G) {
+      A.st->done = 0;
+      A.st->offset = m2.new_offset;
+    }
+  }
+}
+
+// ------------------------------------------------------ load / unload (HBM)
+// load: balanced digits (natural order) -> weighted, line-transformed,
+// twiddled signal, the state the square sweep expects.
+template <int W, int R, int T>
+__global__ __launch_bounds__(T) void load_lines(PlanArgs A) {
+  constexpr int LP = lpitch(W);
+  __shared__ cplx lds[R * LP];
+  const int tid = threadIdx.x;
+  const uint32_t H = A.H;
+  const uint32_t b0 = blockIdx.x * R;
+  for (int a = tid; a < W; a += T) {
+    const int fa = A.fA[a];
+    const double wa = A.wA[a], iwa = A.iwA[a];
+    for (int r = 0; r < R; r++) {
+      const uint32_t j = 2u * ((uint32_t)a * H + b0 + r);
+      const WordWeight we = word_weight(A, fa, wa, iwa, 2 * (b0 + r));
+      const WordWeight wo = word_weight(A, fa, wa, iwa, 2 * (b0 + r) + 1);
+      lds[r * LP + lpad(a)] = {(double)A.digits[j] * we.w, (double)A.digits[j + 1] * wo.w};
+    }
+  }
+  __syncthreads();
+  fft_lds<W, R, T, false>(lds, A.twW);
+  for (int e = tid; e < R * W; e += T) {
+    const int r = e % R, k1 = e / R;
+    const cplx v = lds[r * LP + lpad(k1)];
+    A.x[(size_t)k1 * H + b0 + r] = cmul(v, twiddle_m(A.twM, (b0 + r) * (uint32_t)k1));
+  }
+}
+
+// unload: inverse of load; digits rounded, rounding error max into err[0].
+template <int W, int R, int T>
+__global__ __launch_bounds__(T) void unload_lines(PlanArgs A) {
+  constexpr int LP = lpitch(W);
+  __shared__ cplx lds[R * LP];
+  __shared__ float red[T / 64];
+  const int tid = threadIdx.x;
+  const uint32_t H = A.H;
+  const uint32_t b0 = blockIdx.x * R;
+  for (int e = tid; e < R * W; e += T) {
+    const int r = e % R, k1 = e / R;
+    cplx v = A.x[(size_t)k1 * H + b0 + r];
+    lds[r * LP + lpad(k1)] = cmulc(v, twiddle_m(A.twM, (b0 + r) * (uint32_t)k1));
+  }
+  __syncthreads();
+  fft_lds<W, R, T, true>(lds, A.twW);
+  float emax = 0.f;
+  const double scale = 1.0 / (double)W;
+  for (int a = tid; a < W; a += T) {
+    const int fa = A.fA[a];
+    const double wa = A.wA[a], iwa = A.iwA[a];
+    for (int r = 0; r < R; r++) {
+      const uint32_t j = 2u * ((uint32_t)a * H + b0 + r);
+      const cplx v = lds[r * LP + lpad(a)];
+      const WordWeight we = word_weight(A, fa, wa, iwa, 2 * (b0 + r));
+      const WordWeight wo = word_weight(A, fa, wa, iwa, 2 * (b0 + r) + 1);
+      const double t0 = v.x * we.iw * scale, t1 = v.y * wo.iw * scale;
+      const double r0 = rint(t0), r1 = rint(t1);
+      emax = fmaxf(emax, fmaxf((float)fabs(t0 - r0), (float)fabs(t1 - r1)));
+      A.digits[j] = (int32_t)r0;
+      A.digits[j + 1] = (int32_t)r1;
+    }
+  }
+  const float m = block_max<T>(emax, red);
+  if (tid == 0) atomicMax(&A.err[0], __float_as_uint(m));
+}
+
+// ----------------------------------------------------------- resident path
+// One workgroup per exponent; the signal (natural order) stays in LDS for
+// `iters` iterations.  Thread t owns words [t*C, (t+1)*C), C = N/T.
+template <int M, int T>
+__global__ __launch_bounds__(T) void resident_kernel(const ResidentDesc *descs) {
+  constexpr int LP = lpitch(M);
+  constexpr int N = 2 * M;
+  constexpr int C = N / T;
+  static_assert(C % 2 == 0 && C >= 2, "words per thread");
+  __shared__ cplx lds[LP];
+  __shared__ int64_t s_carry[T];
+  __shared__ float red[T / 64];
+  __shared__ Minus2 s_m2;
+  const ResidentDesc D = descs[blockIdx.x];
+  if (D.iters == 0) return;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < M; i += T) lds[lpad(i)] = D.x[i];
+  const uint32_t p = D.p, b = p % N, q = p / N;
+  const uint32_t j0 = (uint32_t)tid * C;
+  const uint32_t f0 = (uint32_t)(((uint64_t)N - ((uint64_t)b * j0) % N) % N);
+  uint32_t offset = D.offset;
+  __syncthreads();
+  for (uint32_t it = 0; it < D.iters; it++) {
+    if (tid == 0) s_m2 = minus2_position(offset, p, N);
+    fft_lds<M, 1, T, false>(lds, D.tw);
+    for (int k = tid; k <= M / 2; k += T) {
+      const int kb = (M - k) % M;
+      cplx a = lds[lpad(k)], bb = lds[lpad(kb)];
+      pair_square(a, bb, D.tw[k]);
+      lds[lpad(k)] = a;
+      if (kb != k) lds[lpad(kb)] = bb;
+    }
+    __syncthreads();
+    fft_lds<M, 1, T, true>(lds, D.tw);
+    const Minus2 m2 = s_m2;
+    int32_t dgt[C];
+    int widths[C];
+    float emax = 0.f;
+    int64_t carry = 0;
+    uint32_t f = f0;
+#pragma unroll
+    for (int w = 0; w < C; w += 2) {
+      const cplx v = lds[lpad((j0 + w) / 2)];
+#pragma unroll
+      for (int e = 0; e < 2; e++) {
+        const uint32_t j = j0 + w + e;
+        const double t = (e ? v.y : v.x) * D.iw[j] * D.scale;
+        const double rt = rint(t);
+        emax = fmaxf(emax, (float)fabs(t - rt));
+        int64_t iv = (int64_t)rt;
+        if (j == m2.digit) iv -= m2.value;
+        widths[w + e] = (int)q + (f < b ? 1 : 0);
+        dgt[w + e] = (int32_t)balance(iv + carry, widths[w + e], carry);
+        f = f >= b ? f - b : f + N - b;
+      }
+    }
+    s_carry[tid] = carry;
+    __syncthreads();
+    {
+      const int64_t cin = s_carry[(tid + T - 1) % T];
+      int64_t c1;
+      dgt[0] = (int32_t)balance((int64_t)dgt[0] + cin, widths[0], c1);
+      dgt[1] += (int32_t)c1;
+    }
+#pragma unroll
+    for (int w = 0; w < C; w += 2) {
+      const uint32_t j = j0 + w;
+      lds[lpad(j / 2)] = {(double)dgt[w] * D.w[j], (double)dgt[w + 1] * D.w[j + 1]};
+    }
+    const float m = block_max<T>(emax, red);
+    if (tid == 0) D.err[it] = m;
+    offset = m2.new_offset;
+    __syncthreads();
+  }
+  for (int i = tid; i < M; i += T) D.x[i] = lds[lpad(i)];
+  if (tid == 0) *D.offset_out = offset;
+}
+
+// digits <- rint(x / w) for the resident layout
+__global__ void resident_unload(const ResidentDesc *descs, int32_t *digits, uint32_t *err) {
+  const ResidentDesc D = descs[blockIdx.y];
+  const uint32_t N = 2 * D.m;
+  float emax = 0.f;
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < N; j += gridDim.x * blockDim.x) {
+    const cplx v = D.x[j / 2];
+    const double t = (j & 1 ? v.y : v.x) * D.iw[j];
+    const double r = rint(t);
+    emax = fmaxf(emax, (float)fabs(t - r));
+    digits[(size_t)blockIdx.y * N + j] = (int32_t)r;
+  }
+  emax = wave_max(emax);
+  if ((threadIdx.x & 63) == 0) atomicMax(err, __float_as_uint(emax));
+}
+
+__global__ void resident_load(const ResidentDesc *descs, const int32_t *digits) {
+  const ResidentDesc D = descs[blockIdx.y];
+  const uint32_t M = D.m;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) {
+    const size_t base = (size_t)blockIdx.y * 2 * M;
+    D.x[i] = {(double)digits[base + 2 * i] * D.w[2 * i],
+              (double)digits[base + 2 * i + 1] * D.w[2 * i + 1]};
+  }
+}
+
+__global__ void fill_u32(uint32_t *p, uint32_t v, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// ------------------------------------------------------------- dispatch
+
+#define LL_SQUARE(H) {H, (const void *)square_sweep<H, 256>, 256}
+#define LL_CARRY(W, R) {W, R, (const void *)carry_sweep<W, R, 256>, (const void *)load_lines<W, R, 256>, (const void *)unload_lines<W, R, 256>, 256}
+#define LL_RESIDENT(M, T) {M, (const void *)resident_kernel<M, T>, T}
+
+static const SquareKernel k_square[] = {
+    LL_SQUARE(256), LL_SQUARE(512), LL_SQUARE(1024), LL_SQUARE(2048), LL_SQUARE(4096),
+};
+
+static const CarryKernel k_carry[] = {
+    LL_CARRY(256, 4),  LL_CARRY(512, 4),  LL_CARRY(1024, 4), LL_CARRY(2048, 2),
+    LL_CARRY(384, 4),  LL_CARRY(640, 4),  LL_CARRY(768, 4),  LL_CARRY(896, 4),
+    LL_CARRY(1280, 4), LL_CARRY(1536, 4), LL_CARRY(1792, 4), LL_CARRY(1024, 2),
+    LL_CARRY(512, 8),  LL_CARRY(1024, 8),
+};
+
+static const ResidentKernel k_resident[] = {
+    LL_RESIDENT(256, 64),   LL_RESIDENT(512, 128),  LL_RESIDENT(1024, 256),
+    LL_RESIDENT(2048, 256), LL_RESIDENT(4096, 512), LL_RESIDENT(8192, 1024),
+    LL_RESIDENT(1536, 256), LL_RESIDENT(2560, 256), LL_RESIDENT(3584, 256),
+    LL_RESIDENT(3072, 512), LL_RESIDENT(5120, 512), LL_RESIDENT(6144, 512),
+    LL_RESIDENT(7168, 512),
+};
+
+const SquareKernel *find_square(uint32_t H) {
+  for (const auto &k : k_square)
+    if (k.h == H) return &k;
+  return nullptr;
+}
+
+const CarryKernel *find_carry(uint32_t W, uint32_t R) {
+  for (const auto &k : k_carry)
+    if (k.w == W && k.r == R) return &k;
+  return nullptr;
+}
+
+const CarryKernel *carry_table(size_t *count) {
+  *count = sizeof(k_carry) / sizeof(k_carry[0]);
+  return k_carry;
+}
+
+const ResidentKernel *find_resident(uint32_t M) {
+  for (const auto &k : k_resident)
+    if (k.m == M) return &k;
+  return nullptr;
+}
+
+hipError_t launch_square(const SquareKernel *k, const PlanArgs &A, hipStream_t s) {
+  PlanArgs a = A;
+  void *args[] = {&a};
+  return hipLaunchKernel(k->fn, dim3(A.W / 2 + 1), dim3(k->threads), args, 0, s);
+}
+
+hipError_t launch_carry(const CarryKernel *k, const PlanArgs &A, hipStream_t s) {
+  PlanArgs a = A;
+  void *args[] = {&a};
+  return hipLaunchKernel(k->carry, dim3(A.H / k->r + 1), dim3(k->threads), args, 0, s);
+}
+
+hipError_t launch_load(const CarryKernel *k, const PlanArgs &A, hipStream_t s) {
+  PlanArgs a = A;
+  void *args[] = {&a};
+  return hipLaunchKernel(k->load, dim3(A.H / k->r), dim3(k->threads), args, 0, s);
+}
+
+hipError_t launch_unload(const CarryKernel *k, const PlanArgs &A, hipStream_t s) {
+  PlanArgs a = A;
+  void *args[] = {&a};
+  return hipLaunchKernel(k->unload, dim3(A.H / k->r), dim3(k->threads), args, 0, s);
+}
+
+hipError_t launch_resident(const ResidentKernel *k, const ResidentDesc *d_descs, uint32_t count,
+                           hipStream_t s) {
+  const ResidentDesc *d = d_descs;
+  void *args[] = {&d};
+  return hipLaunchKernel(k->fn, dim3(count), dim3(k->threads), args, 0, s);
+}
+
+hipError_t launch_resident_unload(const ResidentDesc *d_descs, uint32_t count, int32_t *digits,
+                                  uint32_t *err, hipStream_t s) {
+  hipLaunchKernelGGL(resident_unload, dim3(16, count), dim3(256), 0, s, d_descs, digits, err);
+  return hipGetLastError();
+}
+
+hipError_t launch_resident_load(const ResidentDesc *d_descs, uint32_t count, const int32_t *digits,
+                                hipStream_t s) {
+  hipLaunchKernelGGL(resident_load, dim3(16, count), dim3(256), 0, s, d_descs, digits);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill(uint32_t *p, uint32_t v, size_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const unsigned blocks = (unsigned)((n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024);
+  hipLaunchKernelGGL(fill_u32, dim3(blocks), dim3(256), 0, s, p, v, n);
+  return hipGetLastError();
+}
+
+}  // namespace llamd

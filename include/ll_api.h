@@ -1,0 +1,137 @@
+/* C-ABI of the MI355X Lucas-Lehmer engine (libcudalucas_amd.so).
+ *
+ * This is the drop-in boundary for the LL iteration path of
+ * tdulcet/CUDALucas v2.06.  The reference has no library API: its path sits
+ * behind internal functions and process globals.  Each entry point below
+ * names the reference function (CUDALucas.cu:line) whose role it takes.
+ * Plain C types only; every call returns an int status (LL_OK or a negative
+ * LL_ERR_*) instead of exit()ing (the reference exits from
+ * cuda_safecalls.h:43-48), and ll_last_error() explains the last failure.
+ *
+ * Digits crossing the boundary are int32 in natural order, value
+ * sum_j d_j 2^ceil(p j / n) mod 2^p - 1 of the *shifted* residue
+ * s * 2^offset, exactly as the reference's x_int / g_xint arrays
+ * (CUDALucas.cu:1254-1342).
+ */
+#ifndef LL_API_H
+#define LL_API_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LL_OK 0
+#define LL_ERR_ARG (-1)         /* invalid argument */
+#define LL_ERR_UNSUPPORTED (-2) /* FFT length / schedule not available */
+#define LL_ERR_HIP (-3)         /* HIP runtime or device error */
+#define LL_ERR_STATE (-4)       /* call out of order (e.g. iterate before load) */
+#define LL_ERR_ALLOC (-5)       /* device or host allocation failed */
+#define LL_ERR_ROUNDOFF (-6)    /* unload rounding error: state corrupt */
+
+typedef struct ll_ctx ll_ctx;
+
+typedef struct ll_options {
+  int32_t device;      /* HIP device index (-d, CUDALucas.cu:3655-3665) */
+  int32_t threads[2];  /* -threads t1 t2 (CUDALucas.cu:812-945); 0 = default.
+                          Mapped to the carry sweep's lines per workgroup
+                          (t2/32, clamped to {2,4,8}). */
+  uint32_t width;      /* schedule override: line length W (0 = auto) */
+  uint32_t height;     /* schedule override: column length H (0 = auto) */
+  uint32_t lines;      /* schedule override: lines per carry workgroup R */
+  uint32_t flags;      /* LL_OPT_* */
+} ll_options;
+
+#define LL_OPT_FORCE_SWEEPS 1u /* use the two-sweep path even if LDS-resident fits */
+#define LL_OPT_NO_GRAPH 2u     /* launch kernels one by one (no hipGraph replay) */
+
+typedef struct ll_info {
+  uint32_t p, n;
+  uint32_t path; /* 0 = LDS-resident single workgroup, 1 = two HBM sweeps */
+  uint32_t width, height, lines;
+  void *stream;  /* the hipStream_t all work of this context runs on */
+  uint64_t iterations; /* iterations run since the last load */
+} ll_info;
+
+/* Replaces init_threads + alloc_gpu_mem + write_gpu_data + cufftPlan1d
+ * (CUDALucas.cu:812, 1096, 1125, 1239): builds tables and buffers for
+ * exponent p at FFT length n (words). */
+int ll_create(uint32_t p, uint32_t n, const ll_options *opts, ll_ctx **out);
+
+/* Replaces close_lucas / free_gpu (CUDALucas.cu:1400-1419). */
+void ll_destroy(ll_ctx *ctx);
+
+/* Replaces init_x + apply_weights (CUDALucas.cu:216-249, 1254-1278):
+ * loads n int32 digits of the shifted residue with shift `offset`.
+ * Digits should be balanced (ll_balance_digits) for full FFT accuracy. */
+int ll_load_digits(ll_ctx *ctx, const int32_t *digits, uint32_t offset);
+
+/* Replaces `count` calls of lucas_square (CUDALucas.cu:1869-1986): each
+ * iteration doubles the offset mod p, squares, subtracts 2 at bit
+ * offset+1, and records its exact maximum |x - rint(x)| (the reference
+ * samples it every ErrorIterations, CUDALucas.cu:1895-1900).
+ * per_iter_err (may be NULL) receives `count` floats; max_err (may be NULL)
+ * the maximum over them.  A value >= 1.0 flags a carry that overflowed the
+ * 32-bit carry shuttle: the iteration must be treated as failed. */
+int ll_iterate(ll_ctx *ctx, uint32_t count, float *per_iter_err, float *max_err);
+
+/* Replaces cudaMemcpy(x_int, g_xint) (CUDALucas.cu:2773, 2910, 2963, 3112,
+ * 3155): n int32 digits of the current shifted residue and its offset. */
+int ll_read_digits(ll_ctx *ctx, int32_t *digits, uint32_t *offset);
+
+/* Replaces reset_err (CUDALucas.cu:1627-1631): clears the running maximum
+ * error reported by ll_error_max. */
+int ll_reset_error(ll_ctx *ctx);
+float ll_error_max(const ll_ctx *ctx);
+
+int ll_get_info(const ll_ctx *ctx, ll_info *info);
+const char *ll_last_error(const ll_ctx *ctx);
+
+/* Test hook, no reference counterpart (the reference can only wait for a
+ * real failure, CUDALucas.cu:2938-2950, 3043): makes the iteration that is
+ * `after` iterations ahead report `value` as its error (kind 1), or makes
+ * the next ll_iterate fail with LL_ERR_HIP as a device error would (kind 2).
+ * kind 0 clears. */
+int ll_debug_inject(ll_ctx *ctx, uint32_t kind, uint64_t after, float value);
+
+/* ---- host codec (CUDALucas.cu:1280-1342, 1465-1614) ------------------ */
+int ll_digit_width(uint32_t p, uint32_t n, uint32_t j);
+/* unpack_bits (CUDALucas.cu:1285-1311) */
+void ll_unpack_bits(uint32_t p, uint32_t n, const uint32_t *packed, int32_t *digits);
+/* balance_digits (CUDALucas.cu:1313-1342) */
+void ll_balance_digits(uint32_t p, uint32_t n, int32_t *digits);
+/* standardize_digits (CUDALucas.cu:1465-1489), exact for any int32 digits:
+ * full cyclic carry to [0, 2^w) digits of the value mod 2^p-1.
+ * Returns 1 if the value is 0. */
+int ll_standardize_digits(uint32_t p, uint32_t n, int32_t *digits);
+/* pack_bits (CUDALucas.cu:1594-1614): standard digits -> ceil(p/32) words */
+void ll_pack_bits(uint32_t p, uint32_t n, const int32_t *digits, uint32_t *packed);
+/* Unshifted residue (value * 2^-offset mod 2^p-1) as ceil(p/32) words. */
+int ll_residue_words(uint32_t p, uint32_t n, const int32_t *digits, uint32_t offset,
+                     uint32_t *words);
+/* find_residue (CUDALucas.cu:1491-1534): low 64 bits of the unshifted
+ * residue. */
+uint64_t ll_res64(uint32_t p, uint32_t n, const int32_t *digits, uint32_t offset);
+/* init_x for a new test (CUDALucas.cu:1261-1267): 4 * 2^offset. */
+void ll_initial_digits(uint32_t p, uint32_t n, uint32_t offset, int32_t *digits);
+/* checkpoint_checksum (CUDALucas.cu:1712-1733) */
+uint32_t ll_checkpoint_crc(const void *bytes, size_t len);
+
+/* ---- FFT length policy (CUDALucas.cu:947-1094) ----------------------- */
+int ll_fft_length_count(void);
+int ll_fft_length_at(int index); /* words, from the reference's default list */
+/* 1 if this build has kernels for length n */
+int ll_fft_supported(uint32_t n);
+/* choose_fft_length (CUDALucas.cu:1027-1068): *index < 0 selects from the
+ * list by the lb/ub power laws; *index >= count is a manual length.  Lengths
+ * without kernels in this build are skipped upward. Returns n or < 0. */
+int ll_choose_fft_length(uint32_t p, int *index);
+void ll_fft_bounds(uint32_t p, int32_t *lb, int32_t *ub);
+/* fft_from_str (CUDALucas.cu:1070-1094): "4096K", "4M", "4194304"; -1 on error */
+int ll_fft_from_str(const char *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

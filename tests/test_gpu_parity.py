@@ -1,0 +1,204 @@
+"""GPU parity of the HIP LL engine against exact arithmetic.
+
+Every case runs through the C-ABI (libcudalucas_amd.so) on the GPU and is
+compared bit-exactly with GMP (the oracle), with the reference's own Res64
+self-test tables (CUDALucas.cu:2673-2729), or with committed GMP fixtures
+for the large exponents.  Floating point only enters through the reported
+roundoff, which must stay under the reference's 0.35 limit
+(CUDALucas.cu:2616, 2883).
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+import cudalucas_amd as L
+import oracle as O
+from cudalucas_amd import workloads as W
+from conftest import GOLDEN, announce
+
+pytestmark = pytest.mark.gpu
+
+ROUNDOFF_LIMIT = 0.35
+
+
+def sha(words):
+    return hashlib.sha256(np.ascontiguousarray(words, dtype="<u4").tobytes()).hexdigest()
+
+
+def random_state(p, seed):
+    rng = np.random.default_rng(seed)
+    w = rng.integers(0, 1 << 32, O.nwords(p), dtype=np.uint64).astype(np.uint32)
+    if p % 32:
+        w[-1] &= (1 << (p % 32)) - 1
+    return w
+
+
+def rotl(words, p, k):
+    v = O.words_to_int(words)
+    k %= p
+    return O.int_to_words(((v << k) | (v >> (p - k))) & ((1 << p) - 1), p)
+
+
+# ----------------------------------------------------------- small N (LDS)
+
+@pytest.mark.parametrize("p,n,iters,offset", [
+    (W.C1_P, 4096, 500, 0),
+    (W.C1_P, 4096, 500, 12345),
+    (44497, 4096, 300, 777),
+    (22133, 1024, 1000, 5),
+    (86243, 8192, 400, 99),
+    (169409, 8192, 300, 3),
+])
+def test_resident_vs_gmp(p, n, iters, offset):
+    announce("resident p=%d n=%d iters=%d offset=%d" % (p, n, iters, offset))
+    w, r, m = L.run_ll(p, n, iters, offset)
+    g = O.gmp_ll(p, iters)
+    assert r == O.res64(g)
+    assert (w == g).all()
+    assert m < ROUNDOFF_LIMIT
+
+
+def test_c1_full_run_vs_gmp():
+    p, n = W.C1_P, W.C1_N
+    announce("C1 full run p=%d n=%d" % (p, n))
+    with L.LLContext(p, n) as c:
+        c.load_initial(0)
+        m, errs = c.iterate(p - 2, per_iter=True)
+        w, r = c.residue()
+    g = O.gmp_ll(p, p - 2)
+    assert (w == g).all()
+    assert errs.shape == (p - 2,) and float(errs.max()) == pytest.approx(m)
+    assert m < ROUNDOFF_LIMIT
+    assert (errs[40:] > 0).all()  # every iteration reports its own maximum
+
+
+def test_mersenne_prime_44497_is_zero():
+    p, n = W.C1_PRIME_P, 4096
+    announce("M44497 full run")
+    w, r, m = L.run_ll(p, n, p - 2, offset=31337)
+    assert r == 0 and not w.any()
+    assert m < ROUNDOFF_LIMIT
+
+
+# ----------------------------------------------------- two HBM sweeps path
+
+SWEEP_CASES = [
+    # (p, n, W, H, R)
+    (2553659, 131072, 256, 256, 4),
+    (1257787, 131072, 256, 256, 4),
+    (4723783, 262144, 512, 256, 4),
+    (4723783, 262144, 256, 512, 4),
+    (9914521, 524288, 1024, 256, 4),
+    (9914521, 524288, 512, 512, 8),
+    (19535569, 1048576, 256, 2048, 4),
+    (19535569, 1048576, 1024, 512, 2),
+    (19535569, 1048576, 2048, 256, 2),
+]
+
+
+@pytest.mark.parametrize("p,n,w,h,r", SWEEP_CASES)
+def test_sweeps_vs_gmp(p, n, w, h, r):
+    iters = 120
+    announce("sweeps p=%d n=%d W=%d H=%d R=%d" % (p, n, w, h, r))
+    with L.LLContext(p, n, width=w, height=h, lines=r, flags=L.LL_OPT_FORCE_SWEEPS) as c:
+        info = c.info()
+        assert (info["path"], info["width"], info["height"], info["lines"]) == ("sweeps", w, h, r)
+        c.load_initial(4242)
+        m = c.iterate(iters)
+        words, res = c.residue()
+    g = O.gmp_ll(p, iters)
+    assert (words == g).all()
+    assert m < ROUNDOFF_LIMIT
+
+
+@pytest.mark.parametrize("p,n,flags", [
+    (42839, 4096, 0),
+    (2553659, 131072, L.LL_OPT_FORCE_SWEEPS),
+    (2553659, 131072, L.LL_OPT_FORCE_SWEEPS | L.LL_OPT_NO_GRAPH),
+    (9914521, 524288, 0),
+])
+def test_random_state_single_steps(p, n, flags):
+    """Random full-size states (not the low-entropy s_k prefix): one and
+    three iterations from a shifted random value vs GMP."""
+    s = random_state(p, seed=p)
+    off = (p * 7919) % p or 1
+    shifted = rotl(s, p, off)
+    digits = L.balance_digits(p, n, L.unpack_bits(p, n, shifted))
+    announce("random state p=%d n=%d flags=%d" % (p, n, flags))
+    with L.LLContext(p, n, flags=flags) as c:
+        c.load_digits(digits, off)
+        w0, _ = c.residue()
+        assert (w0 == s).all()  # load/unload round trip
+        m1 = c.iterate(1)
+        w1, _ = c.residue()
+        m3 = c.iterate(2)
+        w3, _ = c.residue()
+    e1 = O.gmp_square_sub(p, s, 2)
+    e3 = O.gmp_ll(p, 2, start=e1)
+    assert (w1 == e1).all()
+    assert (w3 == e3).all()
+    assert max(m1, m3) < ROUNDOFF_LIMIT
+
+
+# ------------------------------------------------- reference golden vectors
+
+def _selftest_cases():
+    with open(os.path.join(GOLDEN, "reference_selftest.json")) as f:
+        d = json.load(f)
+    out = []
+    for p, r in d["tests"] + d["testl"]:
+        n, _ = L.choose_fft_length(p) if os.path.exists(L.LIB_PATH) else (0, 0)
+        if 0 < n <= 6291456:
+            out.append((p, int(r, 16), n))
+    return out
+
+
+@pytest.mark.parametrize("p,res64,n", _selftest_cases())
+def test_reference_selftest_res64(p, res64, n):
+    """The reference's -r self-test: Res64 after 10,000 iterations."""
+    announce("selftest p=%d n=%d" % (p, n))
+    with L.LLContext(p, n) as c:
+        c.load_initial(p // 3)
+        m = c.iterate(10000)
+        _, r = c.residue()
+    assert r == res64, "p=%d n=%d got %016x want %016x (maxerr %.4f)" % (p, n, r, res64, m)
+    assert m < 0.4
+
+
+# ---------------------------------------------------- large configurations
+
+def test_c2_77m_vs_gmp_fixture(gmp_fixtures):
+    p, n = W.C2_P, W.C2_N
+    marks = sorted(k[1] for k in gmp_fixtures if k[0] == p)
+    assert marks, "missing GMP fixture for p=%d" % p
+    announce("C2 p=%d n=%d marks=%s" % (p, n, marks))
+    with L.LLContext(p, n) as c:
+        c.load_initial(0)
+        done, worst = 0, 0.0
+        for mk in marks:
+            worst = max(worst, c.iterate(mk - done))
+            done = mk
+            w, r = c.residue()
+            fx = gmp_fixtures[(p, mk)]
+            assert "0x%016x" % r == fx["res64"], "iteration %d" % mk
+            assert sha(w) == fx["sha256"], "iteration %d" % mk
+    print("C2 max roundoff over %d iterations: %.4f" % (done, worst))
+    assert worst < ROUNDOFF_LIMIT
+
+
+@pytest.mark.parametrize("n", W.C3_LENGTHS)
+def test_c3_smooth_lengths_vs_gmp_fixture(n, gmp_fixtures):
+    p = W.c3_prime(n)
+    fx = gmp_fixtures[(p, 200)]
+    announce("C3 p=%d n=%d" % (p, n))
+    with L.LLContext(p, n) as c:
+        c.load_initial(0)
+        m = c.iterate(200)
+        w, r = c.residue()
+    assert "0x%016x" % r == fx["res64"]
+    assert sha(w) == fx["sha256"]
+    print("C3 n=%d p=%d bits/word=%.3f max roundoff %.4f" % (n, p, p / n, m))
+    assert m < 0.45
