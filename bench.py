@@ -1,0 +1,164 @@
+#!/usr/bin/env python3
+"""Benchmark of the LL iteration path on one MI355X.
+
+Workload: the large single exponent -- seeded prime p = 77,005,277 at FFT
+length N = 4,194,304 words (18.36 bits/word), s_0 = 4.  One step is one LL
+iteration (square sweep + carry sweep, hipGraph-replayed).  Prints one JSON
+line: ms per iteration over exactly --steps iterations after --warmup
+untimed ones, the roofline of the dominant kernel (algorithmic bytes
+16 N per sweep launch over its hipEvent-measured average duration), and a
+CPU baseline: the oracle's IBDWT restatement of the same (p, N) on one host
+core over a bounded sample, its residue checked against the GPU's.
+
+    python bench.py [--gpus 1] [--steps 10000] [--warmup 200]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def algorithmic(n):
+    """Bytes and flops of one iteration at FFT length n (words)."""
+    import math
+
+    m = n // 2
+    return 32.0 * n, 10.0 * m * math.log2(m) + 20.0 * n
+
+
+def cpu_baseline(p, n, budget_s):
+    """Oracle IBDWT (1 core) on the same workload; residue vs the GPU."""
+    import numpy as np
+
+    import cudalucas_amd as L
+    import oracle as O
+
+    c = O.IBDWT(p, n)
+    c.load(O.initial_digits(p, n, 0), 0)
+    t0 = time.perf_counter()
+    c.iterate(1)
+    one = time.perf_counter() - t0
+    extra = max(2, min(50, int(budget_s / max(one, 1e-3)) - 1))
+    t0 = time.perf_counter()
+    c.iterate(extra)
+    dt = time.perf_counter() - t0
+    iters = 1 + extra
+    d, off = c.read()
+    cpu_words = O.digits_to_residue(p, n, d, off)
+    c.close()
+    with L.LLContext(p, n) as g:
+        g.load_initial(0)
+        g.iterate(iters)
+        gw, _ = g.residue()
+    return {
+        "value": round(dt * 1000.0 / extra, 3),
+        "unit": "ms/iter",
+        "cores": 1,
+        "kind": "port",
+        "sample": "iterations 2..%d of p=%d at N=%d from s_0=4 (oracle IBDWT, 1 host core, %.1f s)"
+        % (iters, p, n, dt),
+        "residue_matches_gpu": bool((cpu_words == gw).all()),
+    }
+
+
+def load_traffic(name):
+    path = os.path.join(HERE, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--profile-iters", type=int, default=200)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; the LL path of one exponent runs on a "
+              "single MI355X (multi-GPU is sharding by exponent, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import cudalucas_amd as L
+    from cudalucas_amd import workloads as W
+
+    p, n = W.C2_P, W.C2_N
+    ctx = L.LLContext(p, n)
+    info = ctx.info()
+    ctx.load_initial(0)
+    if args.warmup:
+        ctx.iterate(args.warmup)
+    t0 = time.perf_counter()
+    maxerr, errs = ctx.iterate(args.steps, per_iter=True)
+    wall = time.perf_counter() - t0
+    gpu_ms = ctx.last_iterate_ms()
+    sq_ms, ca_ms = ctx.profile_kernels(args.profile_iters)
+    words, res64 = ctx.residue()
+    done_iters = args.warmup + args.steps + args.profile_iters
+    ctx.close()
+
+    b_alg, f_alg = algorithmic(n)
+    ms_per_step = wall * 1000.0 / args.steps
+    gpu_ms_per_iter = gpu_ms / args.steps
+    dom_name, dom_ms = ("carry_sweep", ca_ms) if ca_ms >= sq_ms else ("square_sweep", sq_ms)
+    per_launch_bytes = 16.0 * n  # one read + one write of the N-word signal
+    achieved = per_launch_bytes / (dom_ms * 1e-3) / 1e9
+    traffic = load_traffic("traffic_4m.json")
+    roofline = {
+        "bound": "hbm",
+        "kernel": dom_name,
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic.get(dom_name) if traffic else None,
+        "avg_launch_ms": {"square_sweep": round(sq_ms, 5), "carry_sweep": round(ca_ms, 5)},
+    }
+    out = {
+        "metric": "ms/iter at 4M FFT (p~77M)",
+        "value": round(ms_per_step, 5),
+        "unit": "ms/iter",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "large single exponent: seeded prime p=%d, FFT length N=%d words "
+            "(%.2f bits/word), s_0=4" % (p, n, p / n),
+            "p": p,
+            "n": n,
+            "schedule": "%s W=%d H=%d R=%d" % (info["path"], info["width"], info["height"], info["lines"]),
+        },
+        "gpu_ms_per_iter": round(gpu_ms_per_iter, 5),
+        "iter_per_s": round(1000.0 / ms_per_step, 1),
+        "hbm_gbs_alg": round(b_alg / (gpu_ms_per_iter * 1e-3) / 1e9, 1),
+        "hbm_frac_alg": round(b_alg / (gpu_ms_per_iter * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "fp64_util": round(f_alg / (gpu_ms_per_iter * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4),
+        "max_roundoff": round(float(maxerr), 5),
+        "res64": "0x%016x" % res64,
+        "res64_iteration": done_iters,
+        "roofline": roofline,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(p, n, args.cpu_budget)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -96,6 +96,9 @@ def lib():
     L.ll_last_error.argtypes = [P]
     L.ll_last_error.restype = ctypes.c_char_p
     L.ll_debug_inject.argtypes = [P, u32, u64, f32]
+    L.ll_last_iterate_ms.argtypes = [P]
+    L.ll_last_iterate_ms.restype = ctypes.c_double
+    L.ll_profile_kernels.argtypes = [P, u32, P]
     L.ll_digit_width.argtypes = [u32, u32, u32]
     L.ll_unpack_bits.argtypes = [u32, u32, P, P]
     L.ll_unpack_bits.restype = None
@@ -284,6 +287,16 @@ class LLContext:
         d, off = self.read_digits()
         w = residue_words(self.p, self.n, d, off)
         return w, int(w[0]) | (int(w[1]) << 32)
+
+    def last_iterate_ms(self):
+        """GPU time of the kernels of the last iterate() (hipEvents)."""
+        return lib().ll_last_iterate_ms(self._h)
+
+    def profile_kernels(self, count):
+        """Average per-launch ms of (square sweep, carry sweep)."""
+        out = np.zeros(2, dtype=np.float32)
+        self._check(lib().ll_profile_kernels(self._h, count, _ptr(out)))
+        return float(out[0]), float(out[1])
 
     def error_max(self):
         return lib().ll_error_max(self._h)

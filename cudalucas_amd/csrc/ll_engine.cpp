@@ -110,6 +110,9 @@ struct ll_ctx {
   float *r_err = nullptr;
   uint32_t *r_offset = nullptr;
   std::vector<uint32_t> host_ring;
+  // timing (hipEvents on `stream`)
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  double last_gpu_ms = 0.0;
 };
 
 static int fail(ll_ctx *c, int code, const char *fmt, ...) {
@@ -232,6 +235,8 @@ static int build_resident(ll_ctx *c) {
 }
 
 static void release(ll_ctx *c) {
+  for (auto &e : c->ev)
+    if (e) (void)hipEventDestroy(e);
   if (c->graph) (void)hipGraphExecDestroy(c->graph);
   c->graph = nullptr;
   for (void *p : c->allocs) (void)hipFree(p);
@@ -270,6 +275,7 @@ int ll_create(uint32_t p, uint32_t n, const ll_options *opts, ll_ctx **out) {
   *out = nullptr;
   if (p < 3 || n < 512 || n % 2 || (uint64_t)p < 4ull * n / 3) return LL_ERR_ARG;
   if ((double)p / n > 31.0) return LL_ERR_ARG;  // digits must fit int32 with headroom
+  if (p / n < 2) return LL_ERR_ARG;              // digits of at least 2 bits
   ll_ctx *c = new ll_ctx();
   c->p = p;
   c->n = n;
@@ -287,6 +293,10 @@ int ll_create(uint32_t p, uint32_t n, const ll_options *opts, ll_ctx **out) {
   if (!rc) {
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) rc = fail(c, LL_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+  }
+  for (int i = 0; i < 4 && !rc; i++) {
+    e = hipEventCreate(&c->ev[i]);
+    if (e != hipSuccess) rc = fail(c, LL_ERR_HIP, "hipEventCreate: %s", hipGetErrorString(e));
   }
   if (!rc) rc = dalloc(c, &c->x, c->m);
   if (!rc) rc = dalloc(c, &c->digits, n);
@@ -330,17 +340,23 @@ static int iterate_resident(ll_ctx *c, uint32_t count, float *errs) {
   D.offset = c->offset;
   D.iters = count;
   HIPCHK(c, hipMemcpyAsync(c->d_rdesc, &D, sizeof(ResidentDesc), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
   HIPCHK(c, launch_resident(c->sch.rk, c->d_rdesc, 1, c->stream));
+  HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
   HIPCHK(c, hipMemcpyAsync(errs, c->r_err, sizeof(float) * count, hipMemcpyDeviceToHost, c->stream));
   uint32_t off = 0;
   HIPCHK(c, hipMemcpyAsync(&off, c->r_offset, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  float ms = 0.f;
+  HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+  c->last_gpu_ms += ms;
   c->offset = off;
   return LL_OK;
 }
 
 static int iterate_sweeps(ll_ctx *c, uint32_t count, float *errs) {
   HIPCHK(c, launch_fill(c->ring, 0u, kRingSize, c->stream));
+  HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
   uint32_t left = count;
   if (c->use_graph && count >= kGraphIters) {
     int rc = build_graph(c);
@@ -354,11 +370,15 @@ static int iterate_sweeps(ll_ctx *c, uint32_t count, float *errs) {
     HIPCHK(c, launch_square(c->sch.sk, c->args, c->stream));
     HIPCHK(c, launch_carry(c->sch.ck, c->args, c->stream));
   }
+  HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
   HIPCHK(c, hipMemcpyAsync(c->host_ring.data(), c->ring, sizeof(uint32_t) * kRingSize,
                            hipMemcpyDeviceToHost, c->stream));
   uint32_t off = 0;
   HIPCHK(c, hipMemcpyAsync(&off, &c->st->offset, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  float ms = 0.f;
+  HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+  c->last_gpu_ms += ms;
   for (uint32_t i = 0; i < count; i++) {
     const uint32_t bits = c->host_ring[(c->epoch + i) & (kRingSize - 1)];
     memcpy(&errs[i], &bits, 4);
@@ -377,6 +397,7 @@ int ll_iterate(ll_ctx *c, uint32_t count, float *per_iter, float *max_err) {
     return fail(c, LL_ERR_HIP, "injected device error");
   }
   std::vector<float> errs(std::min<uint32_t>(count, kRingSize));
+  c->last_gpu_ms = 0.0;
   float m = 0.f;
   uint32_t done = 0;
   while (done < count) {
@@ -425,6 +446,52 @@ int ll_read_digits(ll_ctx *c, int32_t *digits, uint32_t *offset) {
   memcpy(&e, &ebits, 4);
   if (!(e < 0.05f)) return fail(c, LL_ERR_ROUNDOFF, "digit readback rounding error %g", (double)e);
   if (offset) *offset = c->offset;
+  return LL_OK;
+}
+
+double ll_last_iterate_ms(const ll_ctx *c) { return c ? c->last_gpu_ms : -1.0; }
+
+int ll_profile_kernels(ll_ctx *c, uint32_t count, float *kernel_ms) {
+  if (!c || !kernel_ms || count == 0) return LL_ERR_ARG;
+  if (!c->loaded) return fail(c, LL_ERR_STATE, "ll_profile_kernels before ll_load_digits");
+  HIPCHK(c, hipSetDevice(c->device));
+  kernel_ms[0] = kernel_ms[1] = 0.f;
+  if (c->sch.path == 0) {
+    std::vector<float> errs(std::min<uint32_t>(count, kRingSize));
+    for (uint32_t done = 0; done < count;) {
+      const uint32_t chunk = std::min<uint32_t>(count - done, kRingSize);
+      c->last_gpu_ms = 0.0;
+      int rc = iterate_resident(c, chunk, errs.data());
+      if (rc) return rc;
+      kernel_ms[0] += (float)c->last_gpu_ms;
+      done += chunk;
+    }
+    kernel_ms[0] /= (float)count;  // per iteration inside the resident kernel
+    c->iterations += count;
+    return LL_OK;
+  }
+  HIPCHK(c, launch_fill(c->ring, 0u, kRingSize, c->stream));
+  double sq = 0.0, ca = 0.0;
+  for (uint32_t i = 0; i < count; i++) {
+    HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    HIPCHK(c, launch_square(c->sch.sk, c->args, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+    HIPCHK(c, launch_carry(c->sch.ck, c->args, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev[2]));
+    float a = 0.f, b = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+    HIPCHK(c, hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
+    sq += a;
+    ca += b;
+  }
+  uint32_t off = 0;
+  HIPCHK(c, hipMemcpy(&off, &c->st->offset, sizeof(uint32_t), hipMemcpyDeviceToHost));
+  c->offset = off;
+  c->epoch += count;
+  c->iterations += count;
+  kernel_ms[0] = (float)(sq / count);
+  kernel_ms[1] = (float)(ca / count);
   return LL_OK;
 }
 

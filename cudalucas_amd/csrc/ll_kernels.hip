@@ -292,18 +292,19 @@ __global__ __launch_bounds__(T) void carry_sweep(PlanArgs A) {
       const int32_t cin = (int32_t)(uint32_t)word;
       const int fa = A.fA[a];
       const double wa = A.wA[a], iwa = A.iwA[a];
-      {
-        const WordWeight w0 = word_weight(A, fa, wa, iwa, 2 * b0);
-        int64_t c1;
-        const int64_t d0 = balance((int64_t)dg[c][0] + cin, w0.width, c1);
-        dg[c][0] = (int32_t)d0;
-        dg[c][1] += (int32_t)c1;
-      }
+      // Propagate the incoming carry through the whole chain: what is left
+      // after one word is about sqrt(N)/12 whatever the digit size, which
+      // exceeds half a digit at low bits/word.  Anything left after 2R
+      // words (0 in practice) stays in the last word.
+      int64_t cc = cin;
 #pragma unroll
       for (int r = 0; r < R; r++) {
         const WordWeight we = word_weight(A, fa, wa, iwa, 2 * (b0 + r));
         const WordWeight wo = word_weight(A, fa, wa, iwa, 2 * (b0 + r) + 1);
-        lds[r * LP + lpad(a)] = {(double)dg[c][2 * r] * we.w, (double)dg[c][2 * r + 1] * wo.w};
+        const int64_t d0 = balance((int64_t)dg[c][2 * r] + cc, we.width, cc);
+        int64_t d1 = balance((int64_t)dg[c][2 * r + 1] + cc, wo.width, cc);
+        if (r == R - 1) d1 += cc << wo.width;
+        lds[r * LP + lpad(a)] = {(double)d0 * we.w, (double)d1 * wo.w};
       }
     }
   }
@@ -450,10 +451,13 @@ __global__ __launch_bounds__(T) void resident_kernel(const ResidentDesc *descs) 
     s_carry[tid] = carry;
     __syncthreads();
     {
-      const int64_t cin = s_carry[(tid + T - 1) % T];
-      int64_t c1;
-      dgt[0] = (int32_t)balance((int64_t)dgt[0] + cin, widths[0], c1);
-      dgt[1] += (int32_t)c1;
+      // carry-in from the previous thread's words, propagated through this
+      // thread's whole chain (see carry_sweep); the remainder stays in the
+      // last word
+      int64_t cc = s_carry[(tid + T - 1) % T];
+#pragma unroll
+      for (int w = 0; w < C; w++) dgt[w] = (int32_t)balance((int64_t)dgt[w] + cc, widths[w], cc);
+      dgt[C - 1] += (int32_t)(cc << widths[C - 1]);
     }
 #pragma unroll
     for (int w = 0; w < C; w += 2) {

@@ -86,6 +86,18 @@ int ll_reset_error(ll_ctx *ctx);
 float ll_error_max(const ll_ctx *ctx);
 
 int ll_get_info(const ll_ctx *ctx, ll_info *info);
+
+/* GPU time (ms, hipEvents on the context's stream) of the kernels launched
+ * by the last ll_iterate call, excluding host readback. */
+double ll_last_iterate_ms(const ll_ctx *ctx);
+
+/* Per-kernel timing, the analog of kernel_percentages
+ * (CUDALucas.cu:2002-2140): runs `count` iterations launching each kernel
+ * between its own pair of hipEvents and returns the average duration in ms:
+ * kernel_ms[0] = square sweep (or one whole iteration of the LDS-resident
+ * kernel), kernel_ms[1] = carry sweep (0 for the resident path).  Per-
+ * iteration errors are not collected. */
+int ll_profile_kernels(ll_ctx *ctx, uint32_t count, float *kernel_ms);
 const char *ll_last_error(const ll_ctx *ctx);
 
 /* Test hook, no reference counterpart (the reference can only wait for a

@@ -478,6 +478,8 @@ static int fft_plan_init(fft_plan *pl, uint32_t m) {
   uint32_t r = m;
   pl->m = m;
   pl->nrad = 0;
+  pl->tw = NULL;
+  pl->scratch = NULL;
   while (r % 4 == 0) { pl->rad[pl->nrad++] = 4; r /= 4; }
   while (r % 2 == 0) { pl->rad[pl->nrad++] = 2; r /= 2; }
   while (r % 3 == 0) { pl->rad[pl->nrad++] = 3; r /= 3; }

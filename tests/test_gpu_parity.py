@@ -51,6 +51,7 @@ def rotl(words, p, k):
     (22133, 1024, 1000, 5),
     (86243, 8192, 400, 99),
     (169409, 8192, 300, 3),
+    (86243, 16384, 300, 11),  # 5.3 bits/word
 ])
 def test_resident_vs_gmp(p, n, iters, offset):
     announce("resident p=%d n=%d iters=%d offset=%d" % (p, n, iters, offset))
@@ -87,6 +88,7 @@ def test_mersenne_prime_44497_is_zero():
 
 SWEEP_CASES = [
     # (p, n, W, H, R)
+    (756839, 131072, 256, 256, 4),   # 5.8 bits/word: carry-in must cross the whole chain
     (2553659, 131072, 256, 256, 4),
     (1257787, 131072, 256, 256, 4),
     (4723783, 262144, 512, 256, 4),
