@@ -84,6 +84,8 @@ def main():
     ap.add_argument("--profile-iters", type=int, default=200)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-only", action="store_true",
+                    help="only the 64 MiB streaming-copy calibration kernel")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; the LL path of one exponent runs on a "
@@ -92,6 +94,13 @@ def main():
 
     import cudalucas_amd as L
     from cudalucas_amd import workloads as W
+
+    copy_bytes = 64 << 20
+    copy_ms = L.bench_copy(copy_bytes, reps=50)
+    copy_gbs = 2.0 * copy_bytes / (copy_ms * 1e-3) / 1e9
+    if args.copy_only:
+        print(json.dumps({"copy_bytes": copy_bytes, "copy_ms": copy_ms, "copy_gbs": round(copy_gbs, 1)}))
+        return
 
     p, n = W.C2_P, W.C2_N
     ctx = L.LLContext(p, n)
@@ -124,6 +133,7 @@ def main():
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": traffic.get(dom_name) if traffic else None,
         "avg_launch_ms": {"square_sweep": round(sq_ms, 5), "carry_sweep": round(ca_ms, 5)},
+        "copy_kernel_gbs": round(copy_gbs, 1),
     }
     out = {
         "metric": "ms/iter at 4M FFT (p~77M)",

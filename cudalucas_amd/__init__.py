@@ -99,6 +99,7 @@ def lib():
     L.ll_last_iterate_ms.argtypes = [P]
     L.ll_last_iterate_ms.restype = ctypes.c_double
     L.ll_profile_kernels.argtypes = [P, u32, P]
+    L.ll_bench_copy.argtypes = [i32, u64, u32, ctypes.POINTER(f32)]
     L.ll_digit_width.argtypes = [u32, u32, u32]
     L.ll_unpack_bits.argtypes = [u32, u32, P, P]
     L.ll_unpack_bits.restype = None
@@ -306,6 +307,15 @@ class LLContext:
 
     def inject(self, kind, after=0, value=0.0):
         self._check(lib().ll_debug_inject(self._h, kind, after, value))
+
+
+def bench_copy(nbytes, reps=20, device=0):
+    """Average ms of a plain device copy of nbytes (read nbytes + write nbytes)."""
+    ms = ctypes.c_float()
+    rc = lib().ll_bench_copy(device, nbytes, reps, ctypes.byref(ms))
+    if rc:
+        raise LLError(rc, "ll_bench_copy")
+    return ms.value
 
 
 def run_ll(p, n, iterations, offset=0, **kw):

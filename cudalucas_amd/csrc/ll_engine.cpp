@@ -59,9 +59,9 @@ bool pick_schedule(uint32_t n, const ll_options *o, Schedule *s) {
     if (o && o->width && o->width != ck->w) continue;
     if (o && o->height && o->height != H) continue;
     if (want_r && want_r != ck->r) continue;
-    // prefer H = 2048, then R = 4, then W close to H / 2
+    // prefer H = 2048, then R = 8 (128-byte column runs), then W close to H / 2
     double cost = fabs(log2((double)H) - 11.0) * 4.0;
-    cost += ck->r == 4 ? 0.0 : 1.0;
+    cost += ck->r == 8 ? 0.0 : ck->r == 4 ? 1.0 : 2.0;
     cost += fabs(log2((double)ck->w) - log2((double)H / 2.0));
     if (cost < best) {
       best = cost;
@@ -153,12 +153,12 @@ static int upload(ll_ctx *c, T **ptr, const std::vector<T> &v) {
 static int build_sweeps(ll_ctx *c) {
   const uint32_t p = c->p, n = c->n, M = c->m, W = c->sch.W, H = c->sch.H, R = c->sch.R;
   PlanArgs &A = c->args;
-  std::vector<double> tw;
+  std::vector<double> qc;
   int rc;
-  roots_table(W, tw);
-  if ((rc = upload(c, (double **)&A.twW, tw))) return rc;
-  roots_table(H, tw);
-  if ((rc = upload(c, (double **)&A.twH, tw))) return rc;
+  quarter_cosines(W, qc);
+  if ((rc = upload(c, (double **)&A.qcW, qc))) return rc;
+  quarter_cosines(H, qc);
+  if ((rc = upload(c, (double **)&A.qcH, qc))) return rc;
   // two-level roots of order M
   const uint32_t shift = std::min<uint32_t>(kTwoLevelShift, (uint32_t)log2((double)M));
   const uint32_t S = 1u << shift;
@@ -199,6 +199,8 @@ static int build_sweeps(ll_ctx *c) {
   A.err_mask = kRingSize - 1;
   A.digits = c->digits;
   A.scale = 0.25 / (double)M;
+  A.base_lo = ldexp(1.0, (int)(p / n));
+  A.rshift = (uint32_t)log2((double)R);
   A.p = p;
   A.n = n;
   A.W = W;
@@ -211,14 +213,14 @@ static int build_sweeps(ll_ctx *c) {
 
 static int build_resident(ll_ctx *c) {
   const uint32_t p = c->p, n = c->n, M = c->m;
-  std::vector<double> w(n), iw(n), tw;
+  std::vector<double> w(n), iw(n), qc;
   for (uint32_t j = 0; j < n; j++) weight_pair(weight_exponent(p, n, j), n, &w[j], &iw[j]);
-  roots_table(M, tw);
+  quarter_cosines(M, qc);
   ResidentDesc &D = c->rdesc;
   int rc;
   if ((rc = upload(c, (double **)&D.w, w))) return rc;
   if ((rc = upload(c, (double **)&D.iw, iw))) return rc;
-  if ((rc = upload(c, (double **)&D.tw, tw))) return rc;
+  if ((rc = upload(c, (double **)&D.qc, qc))) return rc;
   if ((rc = dalloc(c, &c->r_err, kRingSize))) return rc;
   if ((rc = dalloc(c, &c->r_offset, 1))) return rc;
   if ((rc = dalloc(c, &c->d_rdesc, 1))) return rc;
@@ -493,6 +495,36 @@ int ll_profile_kernels(ll_ctx *c, uint32_t count, float *kernel_ms) {
   kernel_ms[0] = (float)(sq / count);
   kernel_ms[1] = (float)(ca / count);
   return LL_OK;
+}
+
+int ll_bench_copy(int32_t device, uint64_t bytes, uint32_t reps, float *ms) {
+  if (!ms || bytes < 16 || reps == 0) return LL_ERR_ARG;
+  if (hipSetDevice(device) != hipSuccess) return LL_ERR_HIP;
+  const size_t n = bytes / 16;
+  cplx *a = nullptr, *b = nullptr;
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = LL_OK;
+  if (hipMalloc(&a, n * 16) != hipSuccess || hipMalloc(&b, n * 16) != hipSuccess) rc = LL_ERR_ALLOC;
+  if (!rc && (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess ||
+              hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = LL_ERR_HIP;
+  if (!rc && hipMemsetAsync(a, 0, n * 16, s) != hipSuccess) rc = LL_ERR_HIP;
+  if (!rc && launch_copy(a, b, n, s) != hipSuccess) rc = LL_ERR_HIP;  // warm-up
+  if (!rc && hipEventRecord(e0, s) != hipSuccess) rc = LL_ERR_HIP;
+  for (uint32_t i = 0; !rc && i < reps; i++)
+    if (launch_copy(i & 1 ? b : a, i & 1 ? a : b, n, s) != hipSuccess) rc = LL_ERR_HIP;
+  if (!rc && hipEventRecord(e1, s) != hipSuccess) rc = LL_ERR_HIP;
+  if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = LL_ERR_HIP;
+  float t = 0.f;
+  if (!rc && hipEventElapsedTime(&t, e0, e1) != hipSuccess) rc = LL_ERR_HIP;
+  *ms = t / (float)reps;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (s) (void)hipStreamDestroy(s);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  return rc;
 }
 
 int ll_reset_error(ll_ctx *c) {

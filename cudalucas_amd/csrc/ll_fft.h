@@ -240,12 +240,34 @@ constexpr int plan_radix(int L, int pass) {
 __device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
 __host__ __device__ constexpr int lpitch(int L) { return L + L / 16 + 1; }
 
+// Roots exp(-2 pi i e/L), e in [0, L), from a quarter-wave cosine table
+// qc[e] = cos(2 pi e/L), e in [0, L/4], held in LDS (L divisible by 4).
+template <int L>
+struct QuarterRoots {
+  const double *qc;
+  __device__ __forceinline__ cplx operator()(int e) const {
+    constexpr int Q = L / 4;
+    const int q = e / Q, r = e - q * Q;
+    const double c = qc[r], s = qc[Q - r];  // cos, sin of 2 pi r/L
+    // angle q*pi/2 + theta: (cos, sin) rotated by q quarter turns
+    const double cr = q == 0 ? c : q == 1 ? -s : q == 2 ? -c : s;
+    const double si = q == 0 ? s : q == 1 ? c : q == 2 ? -s : -c;
+    return {cr, -si};
+  }
+};
+
+// Copies the quarter-wave table of order L from global memory into LDS.
+template <int L, int T>
+__device__ __forceinline__ void load_quarter(double *dst, const double *__restrict__ src) {
+  for (int i = threadIdx.x; i <= L / 4; i += T) dst[i] = src[i];
+}
+
 // One Stockham pass over NB transforms of length L held at buf[b*LP + lpad(i)].
 // Pass with radix R and span NS reads x[j + i*L/R], multiplies by
 // w^(i*(j mod NS)) with w = exp(-+2 pi i/(NS*R)), applies a length-R DFT and
 // writes to (j/NS)*NS*R + (j mod NS) + u*NS.
-template <int L, int R, int NS, int NB, int T, bool INV>
-__device__ __forceinline__ void stockham_pass(cplx *buf, const cplx *__restrict__ tw) {
+template <int L, int R, int NS, int NB, int T, bool INV, typename TW>
+__device__ __forceinline__ void stockham_pass(cplx *buf, const TW &tw) {
   constexpr int Q = L / R;
   constexpr int NBF = NB * Q;
   constexpr int PER = (NBF + T - 1) / T;
@@ -262,7 +284,7 @@ __device__ __forceinline__ void stockham_pass(cplx *buf, const cplx *__restrict_
       for (int i = 0; i < R; i++) {
         cplx a = buf[b * LP + lpad(j + i * Q)];
         if (NS > 1 && i > 0) {
-          cplx w = tw[(i * k) * (L / (NS * R))];
+          const cplx w = tw((i * k) * (L / (NS * R)));
           a = INV ? cmulc(a, w) : cmul(a, w);
         }
         v[s][i] = a;
@@ -285,8 +307,8 @@ __device__ __forceinline__ void stockham_pass(cplx *buf, const cplx *__restrict_
   __syncthreads();
 }
 
-template <int L, int NB, int T, bool INV, int PASS = 0, int NS = 1>
-__device__ __forceinline__ void fft_lds(cplx *buf, const cplx *__restrict__ tw) {
+template <int L, int NB, int T, bool INV, int PASS = 0, int NS = 1, typename TW>
+__device__ __forceinline__ void fft_lds(cplx *buf, const TW &tw) {
   if constexpr (NS < L) {
     constexpr int R = plan_radix(L, PASS);
     static_assert(R > 0, "bad plan");

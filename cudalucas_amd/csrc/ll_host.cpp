@@ -54,6 +54,15 @@ void roots_table(uint32_t L, std::vector<double> &out) {
   for (uint32_t e = 0; e < L; e++) root_of_unity(e, L, &out[2 * e], &out[2 * e + 1]);
 }
 
+void quarter_cosines(uint32_t L, std::vector<double> &out) {
+  out.resize(L / 4 + 1);
+  for (uint32_t e = 0; e <= L / 4; e++) {
+    double re, im;
+    root_of_unity(e, L, &re, &im);
+    out[e] = re;
+  }
+}
+
 uint32_t weight_exponent(uint32_t p, uint32_t n, uint64_t j) {
   const uint64_t b = p % n;
   return (uint32_t)((n - (b * j) % n) % n);

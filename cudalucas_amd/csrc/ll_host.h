@@ -12,6 +12,8 @@ void root_of_unity(uint64_t e, uint64_t L, double *re, double *im);
 void root_minus_one(uint64_t e, uint64_t L, double *re, double *im);
 // interleaved (re, im) of exp(-2 pi i e / L) for e < L
 void roots_table(uint32_t L, std::vector<double> &out);
+// cos(2 pi e / L) for e in [0, L/4]
+void quarter_cosines(uint32_t L, std::vector<double> &out);
 // f(j) = (-(p mod n) j) mod n, the IBDWT weight exponent of word j
 uint32_t weight_exponent(uint32_t p, uint32_t n, uint64_t j);
 // 2^(f/n) and 2^(-f/n)

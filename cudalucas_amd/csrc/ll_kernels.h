@@ -24,9 +24,9 @@ struct DevState {
 };
 
 struct PlanArgs {
-  cplx *x;                 // M complex, natural order m = a*H + b
-  const cplx *twW;         // exp(-2 pi i e/W), e < W
-  const cplx *twH;         // exp(-2 pi i e/H), e < H
+  cplx *x;                 // M complex; element a*H + b at ((b/R) W + a) R + b mod R
+  const double *qcW;       // cos(2 pi e/W), e <= W/4
+  const double *qcH;       // cos(2 pi e/H), e <= H/4
   TwoLevel twM;            // exp(-2 pi i e/M), e < M
   const double *wA, *iwA;  // 2^(+-fA[a]/n), a < W
   const int32_t *fA;       // (-b * 2H a) mod n
@@ -38,6 +38,8 @@ struct PlanArgs {
   uint32_t err_mask;
   int32_t *digits;         // N words, natural order (load/unload)
   double scale;            // 1 / (4 M) after the squaring round trip
+  double base_lo;          // 2^q, q = floor(p / n): the small digit base
+  uint32_t rshift;         // log2(R): lines are tiled by R in memory
   uint32_t p, n, W, H, q, b;
 };
 
@@ -45,7 +47,7 @@ struct ResidentDesc {
   cplx *x;            // M complex, natural order (weighted words)
   const double *w;    // 2^(f(j)/n)
   const double *iw;   // 2^(-f(j)/n)
-  const cplx *tw;     // exp(-2 pi i e/M), e < M
+  const double *qc;   // cos(2 pi e/M), e <= M/4
   float *err;         // iters floats
   uint32_t *offset_out;
   double scale;       // 1 / (4 M)
@@ -86,5 +88,6 @@ hipError_t launch_resident_unload(const ResidentDesc *d_descs, uint32_t count, i
 hipError_t launch_resident_load(const ResidentDesc *d_descs, uint32_t count, const int32_t *digits,
                                 hipStream_t s);
 hipError_t launch_fill(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
+hipError_t launch_copy(const cplx *in, cplx *out, size_t n, hipStream_t s);
 
 }  // namespace llamd

@@ -7,8 +7,10 @@
 //    signal lives in one workgroup's LDS and the kernel loops over many
 //    iterations without touching HBM (batched: one workgroup per exponent).
 //
-//  * two sweeps over HBM for large N, M = W x H, signal in natural order
-//    (complex element m = (x[2m], x[2m+1]) at address m = a*H + b):
+//  * two sweeps over HBM for large N, M = W x H.  Complex element
+//    m = a*H + b holds words (x[2m], x[2m+1]); lines b are tiled by R, so the
+//    element lives at ((b / R) W + a) R + (b mod R): a carry workgroup's R
+//    lines are one contiguous block, a column is read in runs of R.
 //      square_sweep : per column pair (k1, W-k1): forward H-point DFT,
 //                     real-pairing squaring, inverse H-point DFT.
 //      carry_sweep  : per block of R lines b: inverse twiddle, inverse W-point
@@ -52,6 +54,15 @@ __device__ __forceinline__ int64_t balance(int64_t v, int width, int64_t &carry)
   const int64_t d = ((v + half) & ((half << 1) - 1)) - half;
   carry = (v - d) >> width;
   return d;
+}
+
+// The same split in FP64, exact for |v| < 2^53: every value here is an
+// integer and the base 2^width a power of two, so v * 2^-width, its rint and
+// the fma below are all exact.  Ties round to even, so the digit lies in
+// [-2^(width-1), 2^(width-1)].
+__device__ __forceinline__ double balance_f64(double v, double base, double inv_base, double &carry) {
+  carry = rint(v * inv_base);
+  return fma(-carry, base, v);
 }
 
 __device__ __forceinline__ float wave_max(float v) {
@@ -130,36 +141,43 @@ __device__ __forceinline__ void pair_square(cplx &A, cplx &B, cplx t) {
 }
 
 // ------------------------------------------------------------ square sweep
-// One workgroup per column pair (k1, W - k1), k1 in [0, W/2].  After the
-// forward H-point DFT, element k2 of column k1 holds Z[k1 + W k2]; its
-// partner M - k is (W - k1, H - 1 - k2) for k1 != 0 and (0, H - k2) for
-// k1 = 0.
+// W/2 workgroups: workgroup g in [1, W/2) owns the column pair (g, W - g);
+// workgroup 0 owns the two self-paired columns 0 and W/2.  After the forward
+// H-point DFT, element k2 of column k1 holds Z[k1 + W k2]; its partner
+// M - k is (W - k1, H - 1 - k2) for k1 != 0 and (0, H - k2) for k1 = 0.
 template <int H, int T>
-__global__ __launch_bounds__(T) void square_sweep(PlanArgs A) {
+__global__ __launch_bounds__(T, 2) void square_sweep(PlanArgs A) {
   constexpr int LP = lpitch(H);
   __shared__ cplx lds[2 * LP];
+  __shared__ double qc[H / 4 + 1];
   const int tid = threadIdx.x;
   const uint32_t W = A.W;
-  const uint32_t k1 = blockIdx.x;
-  const uint32_t k1b = (W - k1) % W;
-  const bool pair = k1b != k1;
-  const cplx *col0 = A.x + (size_t)k1 * H;
-  const cplx *col1 = A.x + (size_t)k1b * H;
+  const uint32_t g = blockIdx.x;
+  const uint32_t c0 = g;
+  const uint32_t c1 = g == 0 ? W / 2 : W - g;
+  // column c, element b lives at ((b >> rs) W + c) << rs | (b & rmask)
+  const uint32_t rs = A.rshift, rmask = (1u << rs) - 1;
+  const cplx *col0 = A.x + ((size_t)c0 << rs);
+  const cplx *col1 = A.x + ((size_t)c1 << rs);
+  load_quarter<H, T>(qc, A.qcH);
   for (int i = tid; i < H; i += T) {
-    lds[lpad(i)] = col0[i];
-    lds[LP + lpad(i)] = col1[i];
+    const size_t off = ((size_t)(i >> rs) * W << rs) | (i & rmask);
+    lds[lpad(i)] = col0[off];
+    lds[LP + lpad(i)] = col1[off];
   }
   __syncthreads();
-  fft_lds<H, 2, T, false>(lds, A.twH);
-  if (pair) {
+  const QuarterRoots<H> tw{qc};
+  fft_lds<H, 2, T, false>(lds, tw);
+  if (g != 0) {
     for (int k2 = tid; k2 < H; k2 += T) {
       const int k2b = H - 1 - k2;
       cplx a = lds[lpad(k2)], b = lds[LP + lpad(k2b)];
-      pair_square(a, b, twiddle_m(A.twM, k1 + W * (uint32_t)k2));
+      pair_square(a, b, twiddle_m(A.twM, c0 + W * (uint32_t)k2));
       lds[lpad(k2)] = a;
       lds[LP + lpad(k2b)] = b;
     }
-  } else if (k1 == 0) {
+  } else {
+    // column 0: k2 <-> H - k2 ; column W/2: k2 <-> H - 1 - k2
     for (int k2 = tid; k2 <= H / 2; k2 += T) {
       const int k2b = (H - k2) % H;
       cplx a = lds[lpad(k2)], b = lds[lpad(k2b)];
@@ -167,31 +185,40 @@ __global__ __launch_bounds__(T) void square_sweep(PlanArgs A) {
       lds[lpad(k2)] = a;
       if (k2b != k2) lds[lpad(k2b)] = b;
     }
-  } else {
     for (int k2 = tid; k2 < (H + 1) / 2; k2 += T) {
       const int k2b = H - 1 - k2;
-      cplx a = lds[lpad(k2)], b = lds[lpad(k2b)];
-      pair_square(a, b, twiddle_m(A.twM, k1 + W * (uint32_t)k2));
-      lds[lpad(k2)] = a;
-      if (k2b != k2) lds[lpad(k2b)] = b;
+      cplx a = lds[LP + lpad(k2)], b = lds[LP + lpad(k2b)];
+      pair_square(a, b, twiddle_m(A.twM, W / 2 + W * (uint32_t)k2));
+      lds[LP + lpad(k2)] = a;
+      if (k2b != k2) lds[LP + lpad(k2b)] = b;
     }
   }
   __syncthreads();
-  fft_lds<H, 2, T, true>(lds, A.twH);
-  cplx *out0 = A.x + (size_t)k1 * H;
-  cplx *out1 = A.x + (size_t)k1b * H;
+  fft_lds<H, 2, T, true>(lds, tw);
+  cplx *out0 = A.x + ((size_t)c0 << rs);
+  cplx *out1 = A.x + ((size_t)c1 << rs);
   for (int i = tid; i < H; i += T) {
-    out0[i] = lds[lpad(i)];
-    if (pair) out1[i] = lds[LP + lpad(i)];
+    const size_t off = ((size_t)(i >> rs) * W << rs) | (i & rmask);
+    out0[off] = lds[lpad(i)];
+    out1[off] = lds[LP + lpad(i)];
   }
 }
 
 // ------------------------------------------------------------- carry sweep
+// H/R workgroups, one per block of R lines, admitted in ticket order.  Each
+// publishes the carries out of its block before waiting for those into it,
+// so every wait is on a block admitted earlier -- except block 0, which
+// waits for the last block's carries (the cyclic wrap 2^p = 1) after
+// publishing its own; no block waits on block 0 finishing, so this cannot
+// deadlock whatever the number of co-resident workgroups.
 template <int W, int R, int T>
-__global__ __launch_bounds__(T) void carry_sweep(PlanArgs A) {
+__global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(PlanArgs A) {
   constexpr int LP = lpitch(W);
   constexpr int CPT = (W + T - 1) / T;  // columns per thread
   __shared__ cplx lds[R * LP];
+  __shared__ double qc[W / 4 + 1];
+  __shared__ int fb[2 * R];  // weight factors shared by every column
+  __shared__ double wb[2 * R], iwb[2 * R];
   __shared__ float red[T / 64];
   __shared__ uint64_t s_ticket;
   __shared__ Minus2 s_m2;
@@ -203,123 +230,123 @@ __global__ __launch_bounds__(T) void carry_sweep(PlanArgs A) {
     s_ticket = atomicAdd(&A.st->ticket, 1ull);
     s_m2 = minus2_position(A.st->offset, A.p, A.n);
   }
+  load_quarter<W, T>(qc, A.qcW);
   __syncthreads();
   const uint64_t tk = s_ticket;
-  const uint32_t epoch = (uint32_t)(tk / (G + 1));
-  const uint32_t slot = (uint32_t)(tk % (G + 1));
-  const bool redo = slot == G;
-  const bool head = slot == 0;
-  const uint32_t blk = redo ? 0 : slot;
+  const uint32_t epoch = (uint32_t)(tk / G);
+  const uint32_t blk = (uint32_t)(tk % G);
+  if (tid < 2 * R) {
+    const uint32_t i = 2 * blk * R + tid;
+    fb[tid] = A.fB[i];
+    wb[tid] = A.wB[i];
+    iwb[tid] = A.iwB[i] * A.scale;
+  }
   const uint32_t b0 = blk * R;
   const Minus2 m2 = s_m2;
+  const QuarterRoots<W> tw{qc};
 
   // load R lines and undo the column twiddle
   for (int e = tid; e < R * W; e += T) {
     const int r = e % R, k1 = e / R;
-    cplx v = A.x[(size_t)k1 * H + b0 + r];
+    cplx v = A.x[(size_t)b0 * W + e];
     v = cmulc(v, twiddle_m(A.twM, (b0 + r) * (uint32_t)k1));
     lds[r * LP + lpad(k1)] = v;
   }
   __syncthreads();
-  fft_lds<W, R, T, true>(lds, A.twW);
+  fft_lds<W, R, T, true>(lds, tw);
 
-  // round, subtract, carry along the 2R consecutive words of each column
-  float emax = 0.f;
-  int32_t dg[CPT][2 * R];
+  // Round, subtract, carry along the 2R consecutive words of each column,
+  // in exact FP64 (balance_f64).  The balanced digits go back into the
+  // column's own LDS slots until the carry-in arrives.
+  const double base_lo = A.base_lo, ibase_lo = 1.0 / base_lo;  // 2^q and 2^-q
+  const double dmod = (double)m2.value;
+  double emax = 0.0;
+  for (int a = tid; a < W; a += T) {
+    const int fa = A.fA[a];
+    const double iwa = A.iwA[a];
+    const uint32_t j0 = 2u * ((uint32_t)a * H + b0);
+    double carry = 0.0;
 #pragma unroll
-  for (int c = 0; c < CPT; c++) {
-    const int a = tid + c * T;
-    if (W % T == 0 || a < W) {
-      const int fa = A.fA[a];
-      const double wa = A.wA[a], iwa = A.iwA[a];
-      const uint32_t j0 = 2u * ((uint32_t)a * H + b0);
-      int64_t carry = 0;
+    for (int r = 0; r < R; r++) {
+      const cplx v = lds[r * LP + lpad(a)];
+      double d[2];
 #pragma unroll
-      for (int r = 0; r < R; r++) {
-        const cplx v = lds[r * LP + lpad(a)];
-#pragma unroll
-        for (int e = 0; e < 2; e++) {
-          const WordWeight ww = word_weight(A, fa, wa, iwa, 2 * (b0 + r) + e);
-          const double t = (e ? v.y : v.x) * (ww.iw * A.scale);
-          const double rt = rint(t);
-          emax = fmaxf(emax, (float)fabs(t - rt));
-          int64_t iv = (int64_t)rt;
-          if (j0 + 2 * r + e == m2.digit) iv -= m2.value;
-          dg[c][2 * r + e] = (int32_t)balance(iv + carry, ww.width, carry);
-        }
+      for (int e = 0; e < 2; e++) {
+        const int i = 2 * r + e;
+        int f = fa + fb[i];
+        const bool wrap = f >= (int)A.n;
+        f -= wrap ? (int)A.n : 0;
+        const bool big = f < (int)A.b;
+        const double t = (e ? v.y : v.x) * (iwa * iwb[i] * (wrap ? 2.0 : 1.0));
+        double rt = rint(t);
+        emax = fmax(emax, fabs(t - rt));
+        if (j0 + i == m2.digit) rt -= dmod;
+        d[e] = balance_f64(rt + carry, big ? 2.0 * base_lo : base_lo, big ? 0.5 * ibase_lo : ibase_lo, carry);
       }
-      if (!redo) {
-        const int32_t c32 = (int32_t)carry;
-        if ((int64_t)c32 != carry) emax = 2.0f;  // carry overflow: flagged as a failed iteration
-        const uint64_t word = ((uint64_t)epoch << 32) | (uint32_t)c32;
-        __hip_atomic_store(&A.shuttle[(size_t)blk * W + a], word, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-      }
+      lds[r * LP + lpad(a)] = {d[0], d[1]};
     }
+    const bool fits = fabs(carry) < 2147483648.0;
+    if (!fits) emax = 2.0;  // carry overflow: flagged as a failed iteration
+    const int32_t c32 = fits ? (int32_t)carry : 0;
+    const uint64_t word = ((uint64_t)epoch << 32) | (uint32_t)c32;
+    __hip_atomic_store(&A.shuttle[(size_t)blk * W + a], word, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
   }
 
-  if (head) {
-    // block 0 is written by the redo ticket; this one only fed block 1
-    if (tid == 0) {
-      const uint32_t old = atomicAdd(&A.st->done, 1u);
-      if (old == G) {
-        A.st->done = 0;
-        A.st->offset = m2.new_offset;
+  // carry-in from the previous block (block 0: the last block, one column
+  // to the left), propagated through the whole chain, then reweight
+  for (int a = tid; a < W; a += T) {
+    const size_t src = blk == 0 ? (size_t)(G - 1) * W + (a + W - 1) % W : (size_t)(blk - 1) * W + a;
+    uint64_t word;
+    uint32_t spins = 0;
+    for (;;) {
+      word = __hip_atomic_load(&A.shuttle[src], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if ((uint32_t)(word >> 32) == epoch) break;
+      if (++spins > kSpinLimit) {  // never hang: flag the iteration as failed
+        emax = 3.0;
+        word = 0;
+        break;
       }
+      __builtin_amdgcn_s_sleep(2);
     }
-    return;
-  }
-
-  // carry-in from the previous block (for block 0: the last block, one
-  // column to the left), then reweight
+    // What is left after one word is about sqrt(N)/12 whatever the digit
+    // size, which exceeds half a digit at low bits/word, so the carry runs
+    // through the whole chain; anything left after 2R words (0 in
+    // practice) stays in the last word.
+    double cc = (double)(int32_t)(uint32_t)word;
+    const int fa = A.fA[a];
+    const double wa = A.wA[a];
 #pragma unroll
-  for (int c = 0; c < CPT; c++) {
-    const int a = tid + c * T;
-    if (W % T == 0 || a < W) {
-      const size_t src = redo ? (size_t)(G - 1) * W + (a + W - 1) % W : (size_t)(blk - 1) * W + a;
-      uint64_t word;
-      uint32_t spins = 0;
-      for (;;) {
-        word = __hip_atomic_load(&A.shuttle[src], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if ((uint32_t)(word >> 32) == epoch) break;
-        if (++spins > kSpinLimit) {  // never hang: flag the iteration as failed
-          emax = 3.0f;
-          word = 0;
-          break;
-        }
-        __builtin_amdgcn_s_sleep(2);
-      }
-      const int32_t cin = (int32_t)(uint32_t)word;
-      const int fa = A.fA[a];
-      const double wa = A.wA[a], iwa = A.iwA[a];
-      // Propagate the incoming carry through the whole chain: what is left
-      // after one word is about sqrt(N)/12 whatever the digit size, which
-      // exceeds half a digit at low bits/word.  Anything left after 2R
-      // words (0 in practice) stays in the last word.
-      int64_t cc = cin;
+    for (int r = 0; r < R; r++) {
+      const cplx dv = lds[r * LP + lpad(a)];
+      double x[2];
 #pragma unroll
-      for (int r = 0; r < R; r++) {
-        const WordWeight we = word_weight(A, fa, wa, iwa, 2 * (b0 + r));
-        const WordWeight wo = word_weight(A, fa, wa, iwa, 2 * (b0 + r) + 1);
-        const int64_t d0 = balance((int64_t)dg[c][2 * r] + cc, we.width, cc);
-        int64_t d1 = balance((int64_t)dg[c][2 * r + 1] + cc, wo.width, cc);
-        if (r == R - 1) d1 += cc << wo.width;
-        lds[r * LP + lpad(a)] = {(double)d0 * we.w, (double)d1 * wo.w};
+      for (int e = 0; e < 2; e++) {
+        const int i = 2 * r + e;
+        int f = fa + fb[i];
+        const bool wrap = f >= (int)A.n;
+        f -= wrap ? (int)A.n : 0;
+        const bool big = f < (int)A.b;
+        const double base = big ? 2.0 * base_lo : base_lo;
+        double d = balance_f64((e ? dv.y : dv.x) + cc, base, big ? 0.5 * ibase_lo : ibase_lo, cc);
+        if (i == 2 * R - 1) d = fma(cc, base, d);
+        x[e] = d * (wa * wb[i] * (wrap ? 0.5 : 1.0));
       }
+      lds[r * LP + lpad(a)] = {x[0], x[1]};
     }
   }
   __syncthreads();
-  fft_lds<W, R, T, false>(lds, A.twW);
+  fft_lds<W, R, T, false>(lds, tw);
   for (int e = tid; e < R * W; e += T) {
     const int r = e % R, k1 = e / R;
     const cplx v = lds[r * LP + lpad(k1)];
-    A.x[(size_t)k1 * H + b0 + r] = cmul(v, twiddle_m(A.twM, (b0 + r) * (uint32_t)k1));
+    A.x[(size_t)b0 * W + e] = cmul(v, twiddle_m(A.twM, (b0 + r) * (uint32_t)k1));
   }
-  const float m = block_max<T>(emax, red);
+  const float m = block_max<T>((float)emax, red);
   if (tid == 0) {
     atomicMax(&A.err[epoch & A.err_mask], __float_as_uint(m));
     const uint32_t old = atomicAdd(&A.st->done, 1u);
-    if (old == G) {
+    if (old == G - 1) {
       A.st->done = 0;
       A.st->offset = m2.new_offset;
     }
@@ -333,6 +360,9 @@ template <int W, int R, int T>
 __global__ __launch_bounds__(T) void load_lines(PlanArgs A) {
   constexpr int LP = lpitch(W);
   __shared__ cplx lds[R * LP];
+  __shared__ double qc[W / 4 + 1];
+  load_quarter<W, T>(qc, A.qcW);
+  const QuarterRoots<W> tw{qc};
   const int tid = threadIdx.x;
   const uint32_t H = A.H;
   const uint32_t b0 = blockIdx.x * R;
@@ -347,11 +377,11 @@ __global__ __launch_bounds__(T) void load_lines(PlanArgs A) {
     }
   }
   __syncthreads();
-  fft_lds<W, R, T, false>(lds, A.twW);
+  fft_lds<W, R, T, false>(lds, tw);
   for (int e = tid; e < R * W; e += T) {
     const int r = e % R, k1 = e / R;
     const cplx v = lds[r * LP + lpad(k1)];
-    A.x[(size_t)k1 * H + b0 + r] = cmul(v, twiddle_m(A.twM, (b0 + r) * (uint32_t)k1));
+    A.x[(size_t)b0 * W + e] = cmul(v, twiddle_m(A.twM, (b0 + r) * (uint32_t)k1));
   }
 }
 
@@ -360,17 +390,20 @@ template <int W, int R, int T>
 __global__ __launch_bounds__(T) void unload_lines(PlanArgs A) {
   constexpr int LP = lpitch(W);
   __shared__ cplx lds[R * LP];
+  __shared__ double qc[W / 4 + 1];
   __shared__ float red[T / 64];
+  load_quarter<W, T>(qc, A.qcW);
+  const QuarterRoots<W> tw{qc};
   const int tid = threadIdx.x;
   const uint32_t H = A.H;
   const uint32_t b0 = blockIdx.x * R;
   for (int e = tid; e < R * W; e += T) {
     const int r = e % R, k1 = e / R;
-    cplx v = A.x[(size_t)k1 * H + b0 + r];
+    cplx v = A.x[(size_t)b0 * W + e];
     lds[r * LP + lpad(k1)] = cmulc(v, twiddle_m(A.twM, (b0 + r) * (uint32_t)k1));
   }
   __syncthreads();
-  fft_lds<W, R, T, true>(lds, A.twW);
+  fft_lds<W, R, T, true>(lds, tw);
   float emax = 0.f;
   const double scale = 1.0 / (double)W;
   for (int a = tid; a < W; a += T) {
@@ -402,12 +435,15 @@ __global__ __launch_bounds__(T) void resident_kernel(const ResidentDesc *descs) 
   constexpr int C = N / T;
   static_assert(C % 2 == 0 && C >= 2, "words per thread");
   __shared__ cplx lds[LP];
+  __shared__ double qc[M / 4 + 1];
   __shared__ int64_t s_carry[T];
   __shared__ float red[T / 64];
   __shared__ Minus2 s_m2;
   const ResidentDesc D = descs[blockIdx.x];
   if (D.iters == 0) return;
   const int tid = threadIdx.x;
+  load_quarter<M, T>(qc, D.qc);
+  const QuarterRoots<M> tw{qc};
   for (int i = tid; i < M; i += T) lds[lpad(i)] = D.x[i];
   const uint32_t p = D.p, b = p % N, q = p / N;
   const uint32_t j0 = (uint32_t)tid * C;
@@ -416,16 +452,16 @@ __global__ __launch_bounds__(T) void resident_kernel(const ResidentDesc *descs) 
   __syncthreads();
   for (uint32_t it = 0; it < D.iters; it++) {
     if (tid == 0) s_m2 = minus2_position(offset, p, N);
-    fft_lds<M, 1, T, false>(lds, D.tw);
+    fft_lds<M, 1, T, false>(lds, tw);
     for (int k = tid; k <= M / 2; k += T) {
       const int kb = (M - k) % M;
       cplx a = lds[lpad(k)], bb = lds[lpad(kb)];
-      pair_square(a, bb, D.tw[k]);
+      pair_square(a, bb, tw(k));
       lds[lpad(k)] = a;
       if (kb != k) lds[lpad(kb)] = bb;
     }
     __syncthreads();
-    fft_lds<M, 1, T, true>(lds, D.tw);
+    fft_lds<M, 1, T, true>(lds, tw);
     const Minus2 m2 = s_m2;
     int32_t dgt[C];
     int widths[C];
@@ -499,6 +535,15 @@ __global__ void resident_load(const ResidentDesc *descs, const int32_t *digits) 
   }
 }
 
+// Plain streaming copy (one read + one write per 16 bytes): the calibration
+// kernel for HBM byte counters and achievable bandwidth.
+__global__ __launch_bounds__(256) void copy_stream(const cplx *__restrict__ in,
+                                                   cplx *__restrict__ out, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    out[i] = in[i];
+}
+
 __global__ void fill_u32(uint32_t *p, uint32_t v, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x)
@@ -508,7 +553,8 @@ __global__ void fill_u32(uint32_t *p, uint32_t v, size_t n) {
 // ------------------------------------------------------------- dispatch
 
 #define LL_SQUARE(H) {H, (const void *)square_sweep<H, 256>, 256}
-#define LL_CARRY(W, R) {W, R, (const void *)carry_sweep<W, R, 256>, (const void *)load_lines<W, R, 256>, (const void *)unload_lines<W, R, 256>, 256}
+#define LL_CARRY_T(W, R, T) {W, R, (const void *)carry_sweep<W, R, T>, (const void *)load_lines<W, R, T>, (const void *)unload_lines<W, R, T>, T}
+#define LL_CARRY(W, R) LL_CARRY_T(W, R, 256)
 #define LL_RESIDENT(M, T) {M, (const void *)resident_kernel<M, T>, T}
 
 static const SquareKernel k_square[] = {
@@ -519,12 +565,13 @@ static const CarryKernel k_carry[] = {
     LL_CARRY(256, 4),  LL_CARRY(512, 4),  LL_CARRY(1024, 4), LL_CARRY(2048, 2),
     LL_CARRY(384, 4),  LL_CARRY(640, 4),  LL_CARRY(768, 4),  LL_CARRY(896, 4),
     LL_CARRY(1280, 4), LL_CARRY(1536, 4), LL_CARRY(1792, 4), LL_CARRY(1024, 2),
-    LL_CARRY(512, 8),  LL_CARRY(1024, 8),
+    LL_CARRY(512, 8),  LL_CARRY_T(1024, 8, 512), LL_CARRY(256, 8), LL_CARRY(384, 8),
+    LL_CARRY(640, 8),  LL_CARRY(768, 8), LL_CARRY_T(896, 8, 512),
 };
 
 static const ResidentKernel k_resident[] = {
     LL_RESIDENT(256, 64),   LL_RESIDENT(512, 128),  LL_RESIDENT(1024, 256),
-    LL_RESIDENT(2048, 256), LL_RESIDENT(4096, 512), LL_RESIDENT(8192, 1024),
+    LL_RESIDENT(2048, 256), LL_RESIDENT(4096, 512), LL_RESIDENT(8192, 512),
     LL_RESIDENT(1536, 256), LL_RESIDENT(2560, 256), LL_RESIDENT(3584, 256),
     LL_RESIDENT(3072, 512), LL_RESIDENT(5120, 512), LL_RESIDENT(6144, 512),
     LL_RESIDENT(7168, 512),
@@ -556,13 +603,13 @@ const ResidentKernel *find_resident(uint32_t M) {
 hipError_t launch_square(const SquareKernel *k, const PlanArgs &A, hipStream_t s) {
   PlanArgs a = A;
   void *args[] = {&a};
-  return hipLaunchKernel(k->fn, dim3(A.W / 2 + 1), dim3(k->threads), args, 0, s);
+  return hipLaunchKernel(k->fn, dim3(A.W / 2), dim3(k->threads), args, 0, s);
 }
 
 hipError_t launch_carry(const CarryKernel *k, const PlanArgs &A, hipStream_t s) {
   PlanArgs a = A;
   void *args[] = {&a};
-  return hipLaunchKernel(k->carry, dim3(A.H / k->r + 1), dim3(k->threads), args, 0, s);
+  return hipLaunchKernel(k->carry, dim3(A.H / k->r), dim3(k->threads), args, 0, s);
 }
 
 hipError_t launch_load(const CarryKernel *k, const PlanArgs &A, hipStream_t s) {
@@ -593,6 +640,11 @@ hipError_t launch_resident_unload(const ResidentDesc *d_descs, uint32_t count, i
 hipError_t launch_resident_load(const ResidentDesc *d_descs, uint32_t count, const int32_t *digits,
                                 hipStream_t s) {
   hipLaunchKernelGGL(resident_load, dim3(16, count), dim3(256), 0, s, d_descs, digits);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy(const cplx *in, cplx *out, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(copy_stream, dim3(8192), dim3(256), 0, s, in, out, n);
   return hipGetLastError();
 }
 

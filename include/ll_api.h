@@ -100,6 +100,11 @@ double ll_last_iterate_ms(const ll_ctx *ctx);
 int ll_profile_kernels(ll_ctx *ctx, uint32_t count, float *kernel_ms);
 const char *ll_last_error(const ll_ctx *ctx);
 
+/* Measurement helper, no reference counterpart: average ms of a plain
+ * device-to-device streaming copy of `bytes` (one read + one write), the
+ * calibration for HBM byte counters and achievable bandwidth. */
+int ll_bench_copy(int32_t device, uint64_t bytes, uint32_t reps, float *ms);
+
 /* Test hook, no reference counterpart (the reference can only wait for a
  * real failure, CUDALucas.cu:2938-2950, 3043): makes the iteration that is
  * `after` iterations ahead report `value` as its error (kind 1), or makes

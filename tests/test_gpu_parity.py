@@ -98,6 +98,11 @@ SWEEP_CASES = [
     (19535569, 1048576, 256, 2048, 4),
     (19535569, 1048576, 1024, 512, 2),
     (19535569, 1048576, 2048, 256, 2),
+    (19535569, 1048576, 1024, 512, 8),   # 512-thread carry workgroups
+    (4723783, 262144, 256, 512, 8),
+    (29097221, 1572864, 768, 1024, 8),   # 3 * 2^18 complex points
+    (48495761, 2621440, 640, 2048, 8),   # 5 * 2^18
+    (67895203, 3670016, 896, 2048, 8),   # 7 * 2^18
 ]
 
 
