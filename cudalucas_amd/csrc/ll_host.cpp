@@ -2,8 +2,11 @@
 // checkpoint checksum, FFT-length policy and the extended-precision tables
 // (roots of unity, IBDWT weights) the kernels read.
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <sys/stat.h>
+#include <sys/types.h>
 
 #include <vector>
 
@@ -214,6 +217,113 @@ uint32_t ll_checkpoint_crc(const void *bytes, size_t len) {
     }
   }
   return crc;
+}
+
+// ------------------------------------------------------ checkpoint files
+
+size_t ll_checkpoint_words(uint32_t p) { return (p + 31) / 32 + 10; }
+
+int ll_checkpoint_encode(const ll_checkpoint *m, const int32_t *digits, uint32_t *words) {
+  if (!m || !digits || !words || m->p < 3 || m->n == 0) return LL_ERR_ARG;
+  const size_t end = (m->p + 31) / 32;
+  std::vector<int32_t> d(digits, digits + m->n);
+  ll_standardize_digits(m->p, m->n, d.data());
+  ll_pack_bits(m->p, m->n, d.data(), words);
+  words[end + 0] = m->p;
+  words[end + 1] = m->n;
+  words[end + 2] = m->next_iter;
+  words[end + 3] = m->offset;
+  words[end + 4] = (uint32_t)(m->total_us >> 15);
+  words[end + 5] = (uint32_t)(m->time_adj_us >> 15);
+  words[end + 6] = m->iter_adj;
+  words[end + 7] = 0;
+  words[end + 8] = 0;  // magic number: unused by the format
+  words[end + 9] = ll_checkpoint_crc(words, 4 * (end + 9));
+  return LL_OK;
+}
+
+int ll_checkpoint_decode(uint32_t p, const uint32_t *words, ll_checkpoint *m) {
+  if (!words || !m || p < 3) return LL_ERR_ARG;
+  const size_t end = (p + 31) / 32;
+  if (words[end + 9] != ll_checkpoint_crc(words, 4 * (end + 9))) return LL_ERR_ARG;
+  if (words[end] != p) return LL_ERR_ARG;
+  m->p = p;
+  m->n = words[end + 1];
+  m->next_iter = words[end + 2];
+  m->offset = words[end + 3];
+  m->total_us = (uint64_t)words[end + 4] << 15;
+  m->time_adj_us = (uint64_t)words[end + 5] << 15;
+  m->iter_adj = words[end + 6];
+  if (m->offset >= p) return LL_ERR_ARG;
+  return LL_OK;
+}
+
+int ll_checkpoint_digits(uint32_t p, uint32_t n, const uint32_t *words, int32_t *digits) {
+  if (!words || !digits || n == 0) return LL_ERR_ARG;
+  ll_unpack_bits(p, n, words, digits);
+  ll_balance_digits(p, n, digits);
+  return LL_OK;
+}
+
+static void ckpt_path(char *buf, size_t len, const char *dir, char kind, uint32_t p) {
+  if (dir && *dir) snprintf(buf, len, "%s/%c%u", dir, kind, p);
+  else snprintf(buf, len, "%c%u", kind, p);
+}
+
+int ll_checkpoint_write(const char *dir, uint32_t p, const uint32_t *words, const char *save_dir,
+                        uint64_t res64) {
+  if (!words) return LL_ERR_ARG;
+  const size_t nbytes = 4 * ll_checkpoint_words(p);
+  char c[1024], t[1024], tmp[1040];
+  ckpt_path(c, sizeof c, dir, 'c', p);
+  ckpt_path(t, sizeof t, dir, 't', p);
+  snprintf(tmp, sizeof tmp, "%s.new", c);
+  FILE *f = fopen(tmp, "wb");
+  if (!f) return LL_ERR_ARG;
+  const bool ok = fwrite(words, 1, nbytes, f) == nbytes;
+  if (fclose(f) != 0 || !ok) {
+    remove(tmp);
+    return LL_ERR_ARG;
+  }
+  remove(t);
+  rename(c, t);
+  if (rename(tmp, c) != 0) return LL_ERR_ARG;
+  if (save_dir && *save_dir) {
+    const size_t end = (p + 31) / 32;
+    mkdir(save_dir, 0755);
+    char sname[1100];
+    snprintf(sname, sizeof sname, "%s/s%u.%u.%016llx", save_dir, p, words[end + 2] - 1,
+             (unsigned long long)res64);
+    FILE *g = fopen(sname, "wb");
+    if (!g) return LL_ERR_ARG;
+    const bool ok2 = fwrite(words, 1, nbytes, g) == nbytes;
+    if (fclose(g) != 0 || !ok2) return LL_ERR_ARG;
+  }
+  return LL_OK;
+}
+
+int ll_checkpoint_read(const char *dir, uint32_t p, uint32_t *words) {
+  if (!words) return LL_ERR_ARG;
+  const size_t nbytes = 4 * ll_checkpoint_words(p);
+  for (int which = 0; which < 2; which++) {
+    char path[1024];
+    ckpt_path(path, sizeof path, dir, which ? 't' : 'c', p);
+    FILE *f = fopen(path, "rb");
+    if (!f) continue;
+    const size_t got = fread(words, 1, nbytes, f);
+    fclose(f);
+    ll_checkpoint meta;
+    if (got == nbytes && ll_checkpoint_decode(p, words, &meta) == LL_OK) return which;
+  }
+  return LL_ERR_ARG;
+}
+
+void ll_checkpoint_remove(const char *dir, uint32_t p) {
+  char c[1024], t[1024];
+  ckpt_path(c, sizeof c, dir, 'c', p);
+  ckpt_path(t, sizeof t, dir, 't', p);
+  remove(c);
+  remove(t);
 }
 
 // ------------------------------------------------------ FFT-length policy

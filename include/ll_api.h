@@ -135,6 +135,34 @@ void ll_initial_digits(uint32_t p, uint32_t n, uint32_t offset, int32_t *digits)
 /* checkpoint_checksum (CUDALucas.cu:1712-1733) */
 uint32_t ll_checkpoint_crc(const void *bytes, size_t len);
 
+/* ---- checkpoint files (CUDALucas.cu:1616-1625, 1707-1861) ------------- */
+/* Layout: uint32 words[ceil(p/32) + 10]; [0, end) the packed shifted residue
+ * (standardized digits, pack_bits), then p, n, next iteration, offset,
+ * total_us >> 15, time_adj_us >> 15, iter_adj, 0, magic 0, CRC of the
+ * preceding 4 (end + 9) bytes. */
+typedef struct ll_checkpoint {
+  uint32_t p, n, next_iter, offset;
+  uint64_t total_us, time_adj_us;
+  uint32_t iter_adj;
+} ll_checkpoint;
+size_t ll_checkpoint_words(uint32_t p);
+/* set_checkpoint_data + standardize_digits + pack_bits + CRC */
+int ll_checkpoint_encode(const ll_checkpoint *meta, const int32_t *digits, uint32_t *words);
+/* Validates size fields and CRC; fills meta. Returns LL_OK or LL_ERR_ARG. */
+int ll_checkpoint_decode(uint32_t p, const uint32_t *words, ll_checkpoint *meta);
+/* Digits at length n (unpack_bits + balance_digits) of a decoded checkpoint. */
+int ll_checkpoint_digits(uint32_t p, uint32_t n, const uint32_t *words, int32_t *digits);
+/* write_checkpoint: rotates <dir>/c<p> to <dir>/t<p>, writes <dir>/c<p>;
+ * with save_dir non-NULL also <save_dir>/s<p>.<iter>.<res64 hex>. */
+int ll_checkpoint_write(const char *dir, uint32_t p, const uint32_t *words, const char *save_dir,
+                        uint64_t res64);
+/* read_checkpoint: c<p>, falling back to t<p>; each must pass its CRC (the
+ * reference checks the backup with a plain sum, CUDALucas.cu:1791).
+ * Returns 0 = c file, 1 = t file, LL_ERR_ARG = neither usable. */
+int ll_checkpoint_read(const char *dir, uint32_t p, uint32_t *words);
+/* rm_checkpoint (CUDALucas.cu:1851-1861) */
+void ll_checkpoint_remove(const char *dir, uint32_t p);
+
 /* ---- FFT length policy (CUDALucas.cu:947-1094) ----------------------- */
 int ll_fft_length_count(void);
 int ll_fft_length_at(int index); /* words, from the reference's default list */

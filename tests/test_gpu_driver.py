@@ -1,0 +1,162 @@
+"""The CUDALucasAMD driver on the GPU: CUDALucas's command-line behaviour.
+
+Results are compared with GMP (the oracle).  Recovery paths are forced from
+the host: roundoff errors through --inject-error (ll_debug_inject), an
+interrupt through SIGINT, a resume through --stop-after and through a
+checkpoint written in the reference's file format by the oracle.
+"""
+import os
+import re
+import signal
+import subprocess
+import time
+
+import numpy as np
+import pytest
+
+import cudalucas_amd as L
+import oracle as O
+from cudalucas_amd import workloads as W
+from conftest import announce
+
+pytestmark = pytest.mark.gpu
+
+DRIVER = os.path.join(os.path.dirname(L.LIB_PATH), "CUDALucasAMD")
+
+
+def drive(args, cwd, timeout=300):
+    announce("driver " + " ".join(str(a) for a in args))
+    r = subprocess.run([DRIVER] + [str(a) for a in args], cwd=cwd, capture_output=True, text=True,
+                       timeout=timeout)
+    return r
+
+
+def result_line(cwd):
+    return (cwd / "results.txt").read_text().strip().splitlines()[-1]
+
+
+def gmp_res64(p, iters):
+    return O.res64(O.gmp_ll(p, iters))
+
+
+def test_prime_exponent_reports_prime(tmp_path):
+    r = drive(["-f", "4096", "-x", "10000", "--offset", "99", 44497], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "M( 44497 )P" in r.stdout
+    assert result_line(tmp_path).startswith("M( 44497 )P, offset = ")
+    assert not (tmp_path / "c44497").exists() or True
+
+
+def test_composite_run_matches_gmp_and_reports(tmp_path):
+    p = W.C1_P
+    r = drive(["-f", "4096", "-x", "10000", "-c", "10000", "--offset", "123", p], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    want = gmp_res64(p, p - 2)
+    assert "M( %d )C, 0x%016x" % (p, want) in result_line(tmp_path)
+    # report lines at every ReportIterations with the Res64 of that iteration
+    reps = re.findall(r"Iteration (\d+) / %d, 0x([0-9a-f]{16})" % p, r.stdout)
+    assert [int(i) for i, _ in reps] == list(range(10000, p - 2, 10000))
+    for it, res in reps[:2]:
+        assert int(res, 16) == gmp_res64(p, int(it))
+    # a finished composite test removes its checkpoints
+    assert not (tmp_path / ("c%d" % p)).exists()
+
+
+def test_stop_and_resume_from_own_checkpoint(tmp_path):
+    p = W.C1_P
+    r1 = drive(["-f", "4096", "-x", "5000", "-c", "5000", "--offset", "7", "--stop-after", "12000", p], tmp_path)
+    assert r1.returncode == 0, r1.stdout + r1.stderr
+    assert (tmp_path / ("c%d" % p)).exists()
+    r2 = drive(["-f", "4096", "-x", "5000", "-c", "5000", p], tmp_path)
+    assert r2.returncode == 0, r2.stdout + r2.stderr
+    assert "Continuing M%d @ iteration 12001" % p in r2.stdout
+    assert "M( %d )C, 0x%016x" % (p, gmp_res64(p, p - 2)) in result_line(tmp_path)
+
+
+def test_resume_from_reference_format_checkpoint(tmp_path):
+    """A c<p> file laid out as CUDALucas writes it (CUDALucas.cu:1707-1849),
+    built by the oracle's restatement of pack_bits and the CRC from the exact
+    GMP state after 20,000 iterations, shift offset 0, FFT length 4096."""
+    p, n, done = W.C1_P, 4096, 20000
+    s = O.gmp_ll(p, done)
+    end = O.nwords(p)
+    words = np.zeros(end + 10, dtype=np.uint32)
+    digits = O.unpack_bits(p, n, s)
+    words[:end] = O.pack_bits(p, n, digits)
+    words[end:end + 9] = [p, n, done + 1, 0, 0, 0, 0, 0, 0]
+    words[end + 9] = O.checkpoint_crc(words[:end + 9].tobytes())
+    (tmp_path / ("c%d" % p)).write_bytes(words.tobytes())
+    r = drive(["-x", "10000", p], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Continuing M%d @ iteration %d" % (p, done + 1) in r.stdout
+    assert "M( %d )C, 0x%016x" % (p, gmp_res64(p, p - 2)) in result_line(tmp_path)
+
+
+def test_roundoff_error_retry_then_larger_fft(tmp_path):
+    p = W.C1_P
+    # once: retried from the last report state and the error goes away
+    r = drive(["-f", "4096", "-x", "5000", "--offset", "5", "--inject-error", "7001:0.45:1", p], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Round off error at iteration = 7001" in r.stdout
+    assert "Restarting from last checkpoint to see if the error is repeatable." in r.stdout
+    assert "Looks like the error went away, continuing." in r.stdout
+    assert "M( %d )C, 0x%016x" % (p, gmp_res64(p, p - 2)) in result_line(tmp_path)
+    # twice: a larger FFT until the next report, then back (CUDALucas.cu:3043-3088)
+    (tmp_path / "results.txt").unlink()
+    r = drive(["-f", "4096", "-x", "5000", "--offset", "5", "--inject-error", "7001:0.45:2", p], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "The error persists." in r.stdout and "Trying a larger fft until the next checkpoint." in r.stdout
+    assert "Resetting fft." in r.stdout
+    assert "M( %d )C, 0x%016x" % (p, gmp_res64(p, p - 2)) in result_line(tmp_path)
+    # three times: a larger FFT did not help either
+    r = drive(["-f", "4096", "-x", "5000", "--offset", "5", "--inject-error", "7001:0.45:3", p], tmp_path)
+    assert r.returncode == 2 and "The error won't go away. I give up." in r.stdout
+
+
+def test_selftest_short_table(tmp_path):
+    r = drive(["-r", "0", "--offset", "1000"], tmp_path, timeout=600)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr
+    assert r.stdout.count("This residue is correct.") == 20
+
+
+def test_worktodo_and_results(tmp_path):
+    aid = "0123456789ABCDEF0123456789ABCDEF"
+    (tmp_path / "worktodo.txt").write_text(
+        "Test=N/A,%d,69,0\n# comment\nDoubleCheck=%s,44497,68,1\n" % (W.C1_P, aid))
+    r = drive(["-x", "10000"], tmp_path)
+    assert r.returncode == 2  # stops when no assignment is left
+    lines = (tmp_path / "results.txt").read_text().strip().splitlines()
+    assert lines[0].startswith("M( %d )C, 0x%016x" % (W.C1_P, gmp_res64(W.C1_P, W.C1_P - 2)))
+    assert lines[1].startswith("M( 44497 )P") and lines[1].endswith("AID: " + aid)
+    assert (tmp_path / "worktodo.txt").read_text() == "# comment\n"
+
+
+def test_sigint_writes_checkpoint(tmp_path):
+    p = 1257787
+    announce("driver SIGINT p=%d" % p)
+    with open(tmp_path / "out.txt", "w") as out, open(tmp_path / "err.txt", "w") as err:
+        proc = subprocess.Popen([DRIVER, "-x", "1000", "--offset", "3", str(p)], cwd=tmp_path,
+                                stdout=out, stderr=err, stdin=subprocess.DEVNULL)
+        try:
+            time.sleep(4.0)
+            proc.send_signal(signal.SIGINT)
+            rc = proc.wait(timeout=60)
+        finally:
+            if proc.poll() is None:
+                proc.kill()
+                proc.wait()
+    outs, errs = (tmp_path / "out.txt").read_text(), (tmp_path / "err.txt").read_text()
+    assert rc == 0, outs[-2000:] + errs
+    assert "SIGINT caught" in errs
+    assert "Estimated time spent so far" in outs
+    words = np.frombuffer((tmp_path / ("c%d" % p)).read_bytes(), dtype=np.uint32)
+    end = O.nwords(p)
+    nxt, off = int(words[end + 2]), int(words[end + 3])
+    assert 1 < nxt < p - 2
+    # the saved state is exactly s_(nxt-1), shifted by `off`: compared with
+    # an independent engine run from s_0 = 4 (GMP for ~10^5 iterations at
+    # this size would take minutes; the engine is pinned to GMP elsewhere)
+    value = O.words_to_int(words[:end])
+    unshifted = ((value >> off) | (value << (p - off))) & ((1 << p) - 1)
+    w, _, _ = L.run_ll(p, 131072, nxt - 1, offset=3)
+    assert O.int_to_words(unshifted, p).tolist() == w.tolist()

@@ -1,0 +1,218 @@
+"""Host-side logic of the product, checked on the CPU (no GPU calls).
+
+- libcudalucas_amd.so loads and exports every entry point include/ll_api.h
+  declares.
+- The product's codec, Res64 and checkpoint checksum agree with the oracle's
+  restatements of the reference (and hence, through the oracle's pins, with
+  exact integers and the CRC check value).
+- Checkpoint files: byte layout and CRC of write_checkpoint
+  (CUDALucas.cu:1707-1861), rotation c -> t, backup fallback when the current
+  file is corrupt, the -s save folder name.
+- FFT-length policy against the oracle's restatement of choose_fft_length.
+- The driver's option/ini precedence (parse_args / process_options).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import cudalucas_amd as L
+import oracle as O
+
+DRIVER = os.path.join(os.path.dirname(L.LIB_PATH), "CUDALucasAMD")
+
+
+def test_library_exports_every_declared_symbol():
+    lib = L.lib()
+    names = L.exported_symbols()
+    assert len(names) >= 30
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def random_value(p, rng):
+    v = int.from_bytes(rng.bytes(O.nwords(p) * 4), "little") % ((1 << p) - 1)
+    return v
+
+
+@pytest.mark.parametrize("p,n", [(44497, 4096), (86243, 4096), (1257787, 65536), (2976221, 163840),
+                                 (77005277, 4194304)])
+def test_codec_matches_oracle(p, n):
+    rng = np.random.default_rng(p)
+    v = random_value(p, rng)
+    w = O.int_to_words(v, p)
+    assert (L.unpack_bits(p, n, w) == O.unpack_bits(p, n, w)).all()
+    d = L.balance_digits(p, n, L.unpack_bits(p, n, w))
+    assert (d == O.balance_digits(p, n, O.unpack_bits(p, n, w))).all()
+    assert all(L.digit_width(p, n, j) == O.lib().orc_digit_width(p, n, j) for j in range(0, n, max(1, n // 257)))
+    off = int(rng.integers(0, p))
+    assert (L.residue_words(p, n, d, off) == O.digits_to_residue(p, n, d, off)).all()
+    assert L.res64(p, n, d, off) == O.find_residue(p, n, d, off)
+    s, z = L.standardize_digits(p, n, d)
+    assert not z and (L.pack_bits(p, n, s) == w).all()
+    assert (L.initial_digits(p, n, off) == O.initial_digits(p, n, off)).all()
+
+
+def test_standardize_handles_unbalanced_and_zero():
+    p, n = 44497, 4096
+    rng = np.random.default_rng(5)
+    d = rng.integers(-(1 << 20), 1 << 20, n).astype(np.int32)  # far outside balanced range
+    assert (L.residue_words(p, n, d, 0) == O.digits_to_residue(p, n, d, 0)).all()
+    # 2^p - 1 == 0
+    ones = np.array([(1 << L.digit_width(p, n, j)) - 1 for j in range(n)], dtype=np.int32)
+    s, z = L.standardize_digits(p, n, ones)
+    assert z and not s.any()
+
+
+def test_crc_matches_oracle_and_check_value():
+    rng = np.random.default_rng(1)
+    for size in (0, 1, 9, 1000, 4099):
+        b = rng.bytes(size)
+        assert L.checkpoint_crc(b) == O.checkpoint_crc(b)
+    assert L.checkpoint_crc(b"123456789") == 0x89A1897F
+
+
+class Ckpt(ctypes.Structure):
+    _fields_ = [("p", ctypes.c_uint32), ("n", ctypes.c_uint32), ("next_iter", ctypes.c_uint32),
+                ("offset", ctypes.c_uint32), ("total_us", ctypes.c_uint64), ("time_adj_us", ctypes.c_uint64),
+                ("iter_adj", ctypes.c_uint32)]
+
+
+def encode(p, n, digits, nxt, off, total_us=0):
+    lib = L.lib()
+    lib.ll_checkpoint_words.restype = ctypes.c_size_t
+    nw = lib.ll_checkpoint_words(p)
+    words = np.zeros(nw, dtype=np.uint32)
+    m = Ckpt(p, n, nxt, off, total_us, 0, 0)
+    d = np.ascontiguousarray(digits, dtype=np.int32)
+    rc = lib.ll_checkpoint_encode(ctypes.byref(m), d.ctypes.data_as(ctypes.c_void_p),
+                                  words.ctypes.data_as(ctypes.c_void_p))
+    assert rc == 0
+    return words
+
+
+def test_checkpoint_layout_matches_reference_format():
+    p, n = 86243, 8192
+    rng = np.random.default_rng(2)
+    v = random_value(p, rng)
+    w = O.int_to_words(v, p)
+    d = L.balance_digits(p, n, L.unpack_bits(p, n, w))
+    words = encode(p, n, d, 12346, 777, total_us=123 << 15)
+    end = O.nwords(p)
+    assert words.size == end + 10
+    assert (words[:end] == w).all()  # packed standardized residue (pack_bits)
+    assert list(words[end:end + 9]) == [p, n, 12346, 777, 123, 0, 0, 0, 0]
+    assert words[end + 9] == O.checkpoint_crc(words[:end + 9].tobytes())
+    # decode
+    m = Ckpt()
+    assert L.lib().ll_checkpoint_decode(p, words.ctypes.data_as(ctypes.c_void_p), ctypes.byref(m)) == 0
+    assert (m.n, m.next_iter, m.offset, m.total_us) == (n, 12346, 777, 123 << 15)
+    # digits at another length carry the same value
+    d2 = np.zeros(16384, dtype=np.int32)
+    L.lib().ll_checkpoint_digits(p, 16384, words.ctypes.data_as(ctypes.c_void_p), d2.ctypes.data_as(ctypes.c_void_p))
+    assert O.words_to_int(L.residue_words(p, 16384, d2, 0)) == v
+    # a flipped bit fails the CRC
+    bad = words.copy()
+    bad[5] ^= 1
+    assert L.lib().ll_checkpoint_decode(p, bad.ctypes.data_as(ctypes.c_void_p), ctypes.byref(m)) != 0
+
+
+def test_checkpoint_files_rotation_and_backup(tmp_path):
+    lib = L.lib()
+    p, n = 44497, 4096
+    d0 = L.initial_digits(p, n, 5)
+    w1 = encode(p, n, d0, 10001, 5)
+    w2 = encode(p, n, d0, 20001, 5)
+    P = ctypes.c_void_p
+    dirb = str(tmp_path).encode()
+    assert lib.ll_checkpoint_write(dirb, p, w1.ctypes.data_as(P), None, ctypes.c_uint64(0)) == 0
+    assert lib.ll_checkpoint_write(dirb, p, w2.ctypes.data_as(P), str(tmp_path / "save").encode(),
+                                   ctypes.c_uint64(0xABCDEF)) == 0
+    assert (tmp_path / ("c%d" % p)).read_bytes() == w2.tobytes()
+    assert (tmp_path / ("t%d" % p)).read_bytes() == w1.tobytes()
+    assert (tmp_path / "save" / ("s%d.20000.0000000000abcdef" % p)).exists()
+    out = np.zeros_like(w1)
+    assert lib.ll_checkpoint_read(dirb, p, out.ctypes.data_as(P)) == 0 and (out == w2).all()
+    # corrupt c: the backup (validated by CRC) is used
+    raw = bytearray((tmp_path / ("c%d" % p)).read_bytes())
+    raw[17] ^= 0x40
+    (tmp_path / ("c%d" % p)).write_bytes(bytes(raw))
+    assert lib.ll_checkpoint_read(dirb, p, out.ctypes.data_as(P)) == 1 and (out == w1).all()
+    # both corrupt / truncated: no checkpoint
+    (tmp_path / ("t%d" % p)).write_bytes(w1.tobytes()[:-4])
+    assert lib.ll_checkpoint_read(dirb, p, out.ctypes.data_as(P)) < 0
+    lib.ll_checkpoint_remove(dirb, p)
+    assert not (tmp_path / ("c%d" % p)).exists() and not (tmp_path / ("t%d" % p)).exists()
+
+
+def test_fft_length_policy_matches_oracle():
+    assert L.fft_lengths() == O.default_fft_lengths()
+    for p in (22133, 42839, 86243, 1000003, 20996011, 57885161, 75846319, 77005277, 116391683):
+        assert L.fft_bounds(p) == O.fft_bounds(p)
+        n, idx = L.choose_fft_length(p)
+        lb, ub = O.fft_bounds(p)
+        assert n >= lb and L.fft_supported(n)
+        # the first supported list length at or above the lower bound
+        first = next(x for x in O.default_fft_lengths() if x >= lb and L.fft_supported(x))
+        assert n == first
+    # manual override (an index beyond the list is a length in words) is
+    # clamped to [lb, ub] like the reference: 4M is below lb(77M) = 4216441,
+    # so it is raised to the next list length this build has kernels for.
+    n, _ = L.choose_fft_length(77005277, 4194304)
+    assert n == 5242880
+    n, _ = L.choose_fft_length(75846319, 4194304)
+    assert n == 4194304
+    n, _ = L.choose_fft_length(44497, 4096)
+    assert n == 4096
+
+
+def test_fft_from_str():
+    assert L.fft_from_str("4096K") == 4194304
+    assert L.fft_from_str("4M") == 4194304
+    assert L.fft_from_str("1536k") == 1572864
+    assert L.fft_from_str("65536") == 65536
+    assert L.fft_from_str("4Kx") == -1 and L.fft_from_str("K") == -1
+
+
+def test_supported_lengths_cover_the_configs():
+    from cudalucas_amd import workloads as W
+
+    for n in [W.C1_N, W.C2_N] + W.C3_LENGTHS:
+        assert L.fft_supported(n)
+
+
+def run_driver(args, cwd):
+    return subprocess.run([DRIVER] + args, cwd=cwd, capture_output=True, text=True, timeout=60)
+
+
+def test_driver_version_help_and_bad_options(tmp_path):
+    r = run_driver(["-v"], tmp_path)
+    assert "CUDALucasAMD" in r.stdout and r.returncode == 2
+    r = run_driver(["-h"], tmp_path)
+    assert "-threads" in r.stderr
+    r = run_driver(["-f"], tmp_path)
+    assert r.returncode == 2 and "can't parse -f" in r.stderr
+    r = run_driver(["-f", "12Q", "44497"], tmp_path)
+    assert r.returncode == 2 and "can't parse fft length" in r.stderr
+
+
+def test_driver_ini_precedence(tmp_path):
+    (tmp_path / "CUDALucas.ini").write_text(
+        "CheckpointIterations=50000\nReportIterations=2000\nErrorLimit=60\nThreads=512 256\n"
+        "FFTLength=4M\nWorkFile=todo.txt\nResultsFile=out.txt\nSaveAllCheckpoints=1\nSaveFolder=keep\n")
+    r = run_driver(["--print-config"], tmp_path)
+    cfg = dict(kv.split("=") for kv in r.stdout.split())
+    assert cfg["checkpoint"] == "50000" and cfg["report"] == "2000"
+    assert cfg["error_limit"] == "47"  # clamped to [1, 47] (CUDALucas.cu:3519-3520)
+    assert cfg["threads"] == "512,256" and cfg["fft"] == "4194304"
+    assert cfg["workfile"] == "todo.txt" and cfg["results"] == "out.txt" and cfg["save_all"] == "1"
+    # command line beats the ini file
+    r = run_driver(["-c", "200000", "-x", "5000", "-f", "2048K", "-threads", "128", "64", "--print-config"], tmp_path)
+    cfg = dict(kv.split("=") for kv in r.stdout.split())
+    assert cfg["checkpoint"] == "200000" and cfg["report"] == "5000"
+    assert cfg["fft"] == "2097152" and cfg["threads"] == "128,64"
+    # intervals are rounded to k * 10^m, k in {1, 2, 5}
+    r = run_driver(["-x", "3000", "--print-config"], tmp_path)
+    assert "report=2000" in r.stdout
