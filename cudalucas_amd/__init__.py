@@ -100,6 +100,17 @@ def lib():
     L.ll_last_iterate_ms.restype = ctypes.c_double
     L.ll_profile_kernels.argtypes = [P, u32, P]
     L.ll_bench_copy.argtypes = [i32, u64, u32, ctypes.POINTER(f32)]
+    L.ll_batch_create.argtypes = [i32, u32, P, u32, ctypes.POINTER(P)]
+    L.ll_batch_destroy.argtypes = [P]
+    L.ll_batch_destroy.restype = None
+    L.ll_batch_step.argtypes = [P, u32, P]
+    L.ll_batch_status.argtypes = [P, u32, ctypes.POINTER(u64), ctypes.POINTER(u32), ctypes.POINTER(f32)]
+    L.ll_batch_read_digits.argtypes = [P, u32, P, ctypes.POINTER(u32)]
+    L.ll_batch_last_step_ms.argtypes = [P]
+    L.ll_batch_last_step_ms.restype = ctypes.c_double
+    L.ll_batch_groups.argtypes = [P]
+    L.ll_batch_last_error.argtypes = [P]
+    L.ll_batch_last_error.restype = ctypes.c_char_p
     L.ll_digit_width.argtypes = [u32, u32, u32]
     L.ll_unpack_bits.argtypes = [u32, u32, P, P]
     L.ll_unpack_bits.restype = None
@@ -307,6 +318,83 @@ class LLContext:
 
     def inject(self, kind, after=0, value=0.0):
         self._check(lib().ll_debug_inject(self._h, kind, after, value))
+
+
+class _BatchTest(ctypes.Structure):
+    _fields_ = [("p", ctypes.c_uint32), ("n", ctypes.c_uint32), ("offset", ctypes.c_uint32),
+                ("iterations", ctypes.c_uint32)]
+
+
+class LLBatch:
+    """Many LL tests advanced together (ll_batch_create .. ll_batch_destroy).
+
+    tests: iterable of (p, n, offset, iterations)."""
+
+    MAX_STEP = 1024
+
+    def __init__(self, tests, device=0, flags=0):
+        tests = list(tests)
+        arr = (_BatchTest * len(tests))(*[_BatchTest(*t) for t in tests])
+        h = ctypes.c_void_p()
+        rc = lib().ll_batch_create(device, len(tests), ctypes.cast(arr, ctypes.c_void_p), flags, ctypes.byref(h))
+        self._h = h.value
+        self.tests = tests
+        if rc:
+            msg = lib().ll_batch_last_error(self._h).decode() if self._h else "ll_batch_create"
+            self.close()
+            raise LLError(rc, msg)
+
+    def _check(self, rc):
+        if rc:
+            raise LLError(rc, lib().ll_batch_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ll_batch_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        self.close()
+
+    def step(self, iterations):
+        errs = np.zeros(len(self.tests), dtype=np.float32)
+        self._check(lib().ll_batch_step(self._h, iterations, _ptr(errs)))
+        return errs
+
+    def last_step_ms(self):
+        return lib().ll_batch_last_step_ms(self._h)
+
+    def groups(self):
+        return lib().ll_batch_groups(self._h)
+
+    def status(self, i):
+        done, rem, m = ctypes.c_uint64(), ctypes.c_uint32(), ctypes.c_float()
+        self._check(lib().ll_batch_status(self._h, i, ctypes.byref(done), ctypes.byref(rem), ctypes.byref(m)))
+        return done.value, rem.value, m.value
+
+    def remaining(self):
+        return [self.status(i)[1] for i in range(len(self.tests))]
+
+    def residue(self, i):
+        p, n = self.tests[i][0], self.tests[i][1]
+        d = np.zeros(n, dtype=np.int32)
+        off = ctypes.c_uint32()
+        self._check(lib().ll_batch_read_digits(self._h, i, _ptr(d), ctypes.byref(off)))
+        w = residue_words(p, n, d, off.value)
+        return w, int(w[0]) | (int(w[1]) << 32)
+
+    def run(self, max_step=MAX_STEP):
+        """Steps until every test has finished; returns per-test max roundoff."""
+        worst = np.zeros(len(self.tests), dtype=np.float32)
+        while any(self.remaining()):
+            worst = np.maximum(worst, self.step(max_step))
+        return worst
 
 
 def bench_copy(nbytes, reps=20, device=0):

@@ -1,0 +1,20 @@
+// Carry-sweep / load / unload instantiations: line lengths 256..2048.
+#include "ll_sweeps.h"
+
+namespace llamd {
+
+#define LL_CARRY_T(W, R, T)                                                              \
+  {W, R, (const void *)carry_sweep<W, R, T>, (const void *)load_lines<W, R, T>,          \
+   (const void *)unload_lines<W, R, T>, T}
+#define LL_CARRY(W, R) LL_CARRY_T(W, R, 256)
+
+extern const CarryKernel k_carry_large[] = {
+    LL_CARRY(256, 4),  LL_CARRY(512, 4),  LL_CARRY(1024, 4), LL_CARRY(2048, 2),
+    LL_CARRY(384, 4),  LL_CARRY(640, 4),  LL_CARRY(768, 4),  LL_CARRY(896, 4),
+    LL_CARRY(1280, 4), LL_CARRY(1536, 4), LL_CARRY(1792, 4), LL_CARRY(1024, 2),
+    LL_CARRY(512, 8),  LL_CARRY_T(1024, 8, 512), LL_CARRY(256, 8), LL_CARRY(384, 8),
+    LL_CARRY(640, 8),  LL_CARRY(768, 8), LL_CARRY_T(896, 8, 512),
+};
+extern const size_t k_carry_large_count = sizeof(k_carry_large) / sizeof(k_carry_large[0]);
+
+}  // namespace llamd

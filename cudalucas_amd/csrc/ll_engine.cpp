@@ -101,6 +101,9 @@ struct ll_ctx {
   uint32_t *unload_err = nullptr;
   // sweeps
   PlanArgs args{};
+  PlanArgs *d_args = nullptr;         // device copy of args
+  PlanArgs *d_args_unload = nullptr;  // args with err -> unload_err
+  uint32_t ring_size = kRingSize;
   uint32_t *ring = nullptr;
   DevState *st = nullptr;
   hipGraphExec_t graph = nullptr;
@@ -108,7 +111,7 @@ struct ll_ctx {
   ResidentDesc rdesc{};
   ResidentDesc *d_rdesc = nullptr;
   float *r_err = nullptr;
-  uint32_t *r_offset = nullptr;
+  uint32_t *r_offset = nullptr;  // device shift offset of the resident path
   std::vector<uint32_t> host_ring;
   // timing (hipEvents on `stream`)
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -192,11 +195,11 @@ static int build_sweeps(ll_ctx *c) {
   HIPCHK(c, hipMemset(A.shuttle, 0xff, shuttle * sizeof(uint64_t)));
   if ((rc = dalloc(c, &c->st, 1))) return rc;
   HIPCHK(c, hipMemset(c->st, 0, sizeof(DevState)));
-  if ((rc = dalloc(c, &c->ring, kRingSize))) return rc;
+  if ((rc = dalloc(c, &c->ring, c->ring_size))) return rc;
   A.x = c->x;
   A.st = c->st;
   A.err = c->ring;
-  A.err_mask = kRingSize - 1;
+  A.err_mask = c->ring_size - 1;
   A.digits = c->digits;
   A.scale = 0.25 / (double)M;
   A.base_lo = ldexp(1.0, (int)(p / n));
@@ -207,7 +210,14 @@ static int build_sweeps(ll_ctx *c) {
   A.H = H;
   A.q = p / n;
   A.b = p % n;
-  c->host_ring.resize(kRingSize);
+  A.remaining = nullptr;
+  c->host_ring.resize(c->ring_size);
+  if ((rc = dalloc(c, &c->d_args, 1))) return rc;
+  if ((rc = dalloc(c, &c->d_args_unload, 1))) return rc;
+  HIPCHK(c, hipMemcpy(c->d_args, &A, sizeof(PlanArgs), hipMemcpyHostToDevice));
+  PlanArgs U = A;
+  U.err = c->unload_err;
+  HIPCHK(c, hipMemcpy(c->d_args_unload, &U, sizeof(PlanArgs), hipMemcpyHostToDevice));
   return LL_OK;
 }
 
@@ -221,18 +231,18 @@ static int build_resident(ll_ctx *c) {
   if ((rc = upload(c, (double **)&D.w, w))) return rc;
   if ((rc = upload(c, (double **)&D.iw, iw))) return rc;
   if ((rc = upload(c, (double **)&D.qc, qc))) return rc;
-  if ((rc = dalloc(c, &c->r_err, kRingSize))) return rc;
+  if ((rc = dalloc(c, &c->r_err, c->ring_size))) return rc;
   if ((rc = dalloc(c, &c->r_offset, 1))) return rc;
   if ((rc = dalloc(c, &c->d_rdesc, 1))) return rc;
   D.x = c->x;
   D.err = c->r_err;
-  D.offset_out = c->r_offset;
+  D.offset = c->r_offset;
   D.scale = 0.25 / (double)M;
+  D.remaining = nullptr;
   D.p = p;
   D.m = M;
-  D.offset = 0;
   D.iters = 0;
-  c->host_ring.resize(kRingSize);
+  c->host_ring.resize(c->ring_size);
   return LL_OK;
 }
 
@@ -252,8 +262,8 @@ static int build_graph(ll_ctx *c) {
   hipGraph_t g = nullptr;
   HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   for (uint32_t i = 0; i < kGraphIters; i++) {
-    (void)launch_square(c->sch.sk, c->args, c->stream);
-    (void)launch_carry(c->sch.ck, c->args, c->stream);
+    (void)launch_square(c->sch.sk, c->d_args, 1, c->sch.W, c->stream);
+    (void)launch_carry(c->sch.ck, c->d_args, 1, c->sch.H, c->stream);
   }
   HIPCHK(c, hipStreamEndCapture(c->stream, &g));
   hipError_t e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
@@ -272,7 +282,15 @@ int ll_fft_supported(uint32_t n) {
   return pick_schedule(n, nullptr, &s) ? 1 : 0;
 }
 
+static int ctx_create(uint32_t p, uint32_t n, const ll_options *opts, uint32_t ring_size, ll_ctx **out);
+
 int ll_create(uint32_t p, uint32_t n, const ll_options *opts, ll_ctx **out) {
+  return ctx_create(p, n, opts, kRingSize, out);
+}
+
+}  // extern "C"
+
+static int ctx_create(uint32_t p, uint32_t n, const ll_options *opts, uint32_t ring_size, ll_ctx **out) {
   if (!out) return LL_ERR_ARG;
   *out = nullptr;
   if (p < 3 || n < 512 || n % 2 || (uint64_t)p < 4ull * n / 3) return LL_ERR_ARG;
@@ -282,6 +300,7 @@ int ll_create(uint32_t p, uint32_t n, const ll_options *opts, ll_ctx **out) {
   c->p = p;
   c->n = n;
   c->m = n / 2;
+  c->ring_size = ring_size;
   c->device = opts ? opts->device : 0;
   c->use_graph = !(opts && (opts->flags & LL_OPT_NO_GRAPH));
   if (!pick_schedule(n, opts, &c->sch)) {
@@ -308,6 +327,8 @@ int ll_create(uint32_t p, uint32_t n, const ll_options *opts, ll_ctx **out) {
   return rc;
 }
 
+extern "C" {
+
 void ll_destroy(ll_ctx *c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
@@ -322,12 +343,12 @@ int ll_load_digits(ll_ctx *c, const int32_t *digits, uint32_t offset) {
   offset %= c->p;
   HIPCHK(c, hipMemcpyAsync(c->digits, digits, sizeof(int32_t) * c->n, hipMemcpyHostToDevice, c->stream));
   if (c->sch.path == 0) {
-    c->rdesc.offset = offset;
     c->rdesc.iters = 0;
     HIPCHK(c, hipMemcpyAsync(c->d_rdesc, &c->rdesc, sizeof(ResidentDesc), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->r_offset, &offset, sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, launch_resident_load(c->d_rdesc, 1, c->digits, c->stream));
   } else {
-    HIPCHK(c, launch_load(c->sch.ck, c->args, c->stream));
+    HIPCHK(c, launch_load(c->sch.ck, c->d_args, 1, c->sch.H, c->stream));
     HIPCHK(c, hipMemcpyAsync(&c->st->offset, &offset, sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -339,7 +360,6 @@ int ll_load_digits(ll_ctx *c, const int32_t *digits, uint32_t offset) {
 
 static int iterate_resident(ll_ctx *c, uint32_t count, float *errs) {
   ResidentDesc D = c->rdesc;
-  D.offset = c->offset;
   D.iters = count;
   HIPCHK(c, hipMemcpyAsync(c->d_rdesc, &D, sizeof(ResidentDesc), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
@@ -357,7 +377,7 @@ static int iterate_resident(ll_ctx *c, uint32_t count, float *errs) {
 }
 
 static int iterate_sweeps(ll_ctx *c, uint32_t count, float *errs) {
-  HIPCHK(c, launch_fill(c->ring, 0u, kRingSize, c->stream));
+  HIPCHK(c, launch_fill(c->ring, 0u, c->ring_size, c->stream));
   HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
   uint32_t left = count;
   if (c->use_graph && count >= kGraphIters) {
@@ -369,11 +389,11 @@ static int iterate_sweeps(ll_ctx *c, uint32_t count, float *errs) {
     }
   }
   for (; left; left--) {
-    HIPCHK(c, launch_square(c->sch.sk, c->args, c->stream));
-    HIPCHK(c, launch_carry(c->sch.ck, c->args, c->stream));
+    HIPCHK(c, launch_square(c->sch.sk, c->d_args, 1, c->sch.W, c->stream));
+    HIPCHK(c, launch_carry(c->sch.ck, c->d_args, 1, c->sch.H, c->stream));
   }
   HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-  HIPCHK(c, hipMemcpyAsync(c->host_ring.data(), c->ring, sizeof(uint32_t) * kRingSize,
+  HIPCHK(c, hipMemcpyAsync(c->host_ring.data(), c->ring, sizeof(uint32_t) * c->ring_size,
                            hipMemcpyDeviceToHost, c->stream));
   uint32_t off = 0;
   HIPCHK(c, hipMemcpyAsync(&off, &c->st->offset, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
@@ -382,7 +402,7 @@ static int iterate_sweeps(ll_ctx *c, uint32_t count, float *errs) {
   HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
   c->last_gpu_ms += ms;
   for (uint32_t i = 0; i < count; i++) {
-    const uint32_t bits = c->host_ring[(c->epoch + i) & (kRingSize - 1)];
+    const uint32_t bits = c->host_ring[(c->epoch + i) & (c->ring_size - 1)];
     memcpy(&errs[i], &bits, 4);
   }
   c->epoch += count;
@@ -398,12 +418,12 @@ int ll_iterate(ll_ctx *c, uint32_t count, float *per_iter, float *max_err) {
     c->inject_kind = 0;
     return fail(c, LL_ERR_HIP, "injected device error");
   }
-  std::vector<float> errs(std::min<uint32_t>(count, kRingSize));
+  std::vector<float> errs(std::min<uint32_t>(count, c->ring_size));
   c->last_gpu_ms = 0.0;
   float m = 0.f;
   uint32_t done = 0;
   while (done < count) {
-    const uint32_t chunk = std::min<uint32_t>(count - done, kRingSize);
+    const uint32_t chunk = std::min<uint32_t>(count - done, c->ring_size);
     int rc = c->sch.path == 0 ? iterate_resident(c, chunk, errs.data()) : iterate_sweeps(c, chunk, errs.data());
     if (rc) return rc;
     if (c->inject_kind == 1) {
@@ -429,16 +449,16 @@ int ll_read_digits(ll_ctx *c, int32_t *digits, uint32_t *offset) {
   if (!c->loaded) return fail(c, LL_ERR_STATE, "ll_read_digits before ll_load_digits");
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, launch_fill(c->unload_err, 0u, 1, c->stream));
+  uint32_t dev_offset = 0;
   if (c->sch.path == 0) {
     ResidentDesc D = c->rdesc;
-    D.offset = c->offset;
     D.iters = 0;
     HIPCHK(c, hipMemcpyAsync(c->d_rdesc, &D, sizeof(ResidentDesc), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, launch_resident_unload(c->d_rdesc, 1, c->digits, c->unload_err, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&dev_offset, c->r_offset, 4, hipMemcpyDeviceToHost, c->stream));
   } else {
-    PlanArgs A = c->args;
-    A.err = c->unload_err;
-    HIPCHK(c, launch_unload(c->sch.ck, A, c->stream));
+    HIPCHK(c, launch_unload(c->sch.ck, c->d_args_unload, 1, c->sch.H, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&dev_offset, &c->st->offset, 4, hipMemcpyDeviceToHost, c->stream));
   }
   uint32_t ebits = 0;
   HIPCHK(c, hipMemcpyAsync(digits, c->digits, sizeof(int32_t) * c->n, hipMemcpyDeviceToHost, c->stream));
@@ -447,7 +467,8 @@ int ll_read_digits(ll_ctx *c, int32_t *digits, uint32_t *offset) {
   float e;
   memcpy(&e, &ebits, 4);
   if (!(e < 0.05f)) return fail(c, LL_ERR_ROUNDOFF, "digit readback rounding error %g", (double)e);
-  if (offset) *offset = c->offset;
+  c->offset = dev_offset;
+  if (offset) *offset = dev_offset;
   return LL_OK;
 }
 
@@ -459,9 +480,9 @@ int ll_profile_kernels(ll_ctx *c, uint32_t count, float *kernel_ms) {
   HIPCHK(c, hipSetDevice(c->device));
   kernel_ms[0] = kernel_ms[1] = 0.f;
   if (c->sch.path == 0) {
-    std::vector<float> errs(std::min<uint32_t>(count, kRingSize));
+    std::vector<float> errs(std::min<uint32_t>(count, c->ring_size));
     for (uint32_t done = 0; done < count;) {
-      const uint32_t chunk = std::min<uint32_t>(count - done, kRingSize);
+      const uint32_t chunk = std::min<uint32_t>(count - done, c->ring_size);
       c->last_gpu_ms = 0.0;
       int rc = iterate_resident(c, chunk, errs.data());
       if (rc) return rc;
@@ -472,13 +493,13 @@ int ll_profile_kernels(ll_ctx *c, uint32_t count, float *kernel_ms) {
     c->iterations += count;
     return LL_OK;
   }
-  HIPCHK(c, launch_fill(c->ring, 0u, kRingSize, c->stream));
+  HIPCHK(c, launch_fill(c->ring, 0u, c->ring_size, c->stream));
   double sq = 0.0, ca = 0.0;
   for (uint32_t i = 0; i < count; i++) {
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-    HIPCHK(c, launch_square(c->sch.sk, c->args, c->stream));
+    HIPCHK(c, launch_square(c->sch.sk, c->d_args, 1, c->sch.W, c->stream));
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-    HIPCHK(c, launch_carry(c->sch.ck, c->args, c->stream));
+    HIPCHK(c, launch_carry(c->sch.ck, c->d_args, 1, c->sch.H, c->stream));
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
     HIPCHK(c, hipEventSynchronize(c->ev[2]));
     float a = 0.f, b = 0.f;
@@ -557,5 +578,273 @@ int ll_debug_inject(ll_ctx *c, uint32_t kind, uint64_t after, float value) {
   c->inject_value = value;
   return LL_OK;
 }
+
+}  // extern "C"
+
+// ===================================================================== batch
+// Many LL tests of uneven exponents and lengths on one GPU.  Tests of one
+// FFT length and schedule form a group; a group of two-sweep tests runs as
+// one square and one carry launch per iteration (blockIdx.y = test), a group
+// of LDS-resident tests as one launch per step (one workgroup per test),
+// on a second stream so it overlaps the HBM-bound sweeps.  Per-test
+// device-side "remaining" counters retire finished tests inside replayed
+// hipGraphs without host involvement.
+
+namespace {
+
+constexpr uint32_t kBatchRing = 1024;  // iterations per batch step at most
+
+struct BatchGroup {
+  Schedule sch{};
+  uint32_t n = 0;
+  std::vector<uint32_t> members;  // test indices
+  PlanArgs *d_args = nullptr;     // sweeps: per-member args
+  ResidentDesc *d_desc = nullptr; // resident: per-member descriptors
+  std::vector<ResidentDesc> desc;
+};
+
+}  // namespace
+
+struct ll_batch {
+  int device = 0;
+  hipStream_t stream = nullptr, stream2 = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  std::vector<ll_ctx *> tests;
+  std::vector<BatchGroup> groups;
+  std::vector<void *> allocs;
+  uint32_t *d_remaining = nullptr;
+  std::vector<uint32_t> remaining;   // host copy after the last step
+  std::vector<uint64_t> done;        // iterations run per test
+  std::vector<float> max_err;        // running maximum per test
+  hipGraphExec_t graph = nullptr;
+  double last_ms = 0.0;
+  char msg[256] = {0};
+};
+
+static int bfail(ll_batch *b, int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(b->msg, sizeof(b->msg), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define BCHK(b, expr)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess) return bfail((b), LL_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+static void batch_release(ll_batch *b) {
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  for (auto *c : b->tests) ll_destroy(c);
+  b->tests.clear();
+  for (void *p : b->allocs) (void)hipFree(p);
+  b->allocs.clear();
+  for (auto &e : b->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  if (b->stream2) (void)hipStreamDestroy(b->stream2);
+}
+
+template <typename T>
+static int balloc(ll_batch *b, T **ptr, size_t count) {
+  void *p = nullptr;
+  if (hipMalloc(&p, count * sizeof(T) + 16) != hipSuccess) return bfail(b, LL_ERR_ALLOC, "hipMalloc batch");
+  b->allocs.push_back(p);
+  *ptr = (T *)p;
+  return LL_OK;
+}
+
+static int batch_graph(ll_batch *b) {
+  if (b->graph) return LL_OK;
+  hipGraph_t g = nullptr;
+  BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  for (uint32_t i = 0; i < kGraphIters; i++)
+    for (auto &grp : b->groups)
+      if (grp.sch.path == 1) {
+        (void)launch_square(grp.sch.sk, grp.d_args, (uint32_t)grp.members.size(), grp.sch.W, b->stream);
+        (void)launch_carry(grp.sch.ck, grp.d_args, (uint32_t)grp.members.size(), grp.sch.H, b->stream);
+      }
+  BCHK(b, hipStreamEndCapture(b->stream, &g));
+  hipError_t e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    b->graph = nullptr;
+    return bfail(b, LL_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+  }
+  return LL_OK;
+}
+
+extern "C" {
+
+int ll_batch_create(int32_t device, uint32_t count, const ll_batch_test *tests, uint32_t flags,
+                    ll_batch **out) {
+  if (!out || !tests || count == 0) return LL_ERR_ARG;
+  ll_batch *b = new ll_batch();
+  *out = b;
+  b->device = device;
+  if (hipSetDevice(device) != hipSuccess) return bfail(b, LL_ERR_HIP, "hipSetDevice(%d)", device);
+  BCHK(b, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  BCHK(b, hipStreamCreateWithFlags(&b->stream2, hipStreamNonBlocking));
+  for (auto &e : b->ev) BCHK(b, hipEventCreate(&e));
+  int rc = balloc(b, &b->d_remaining, count);
+  if (rc) return rc;
+  b->remaining.resize(count);
+  b->done.assign(count, 0);
+  b->max_err.assign(count, 0.f);
+  ll_options o;
+  memset(&o, 0, sizeof o);
+  o.device = device;
+  o.flags = flags & LL_OPT_FORCE_SWEEPS;
+  for (uint32_t i = 0; i < count; i++) {
+    ll_ctx *c = nullptr;
+    rc = ctx_create(tests[i].p, tests[i].n, &o, kBatchRing, &c);
+    if (c) b->tests.push_back(c);
+    if (rc) return bfail(b, rc, "test %u (p=%u n=%u): %s", i, tests[i].p, tests[i].n, c ? c->msg : "");
+    std::vector<int32_t> d(tests[i].n);
+    ll_initial_digits(tests[i].p, tests[i].n, tests[i].offset % tests[i].p, d.data());
+    if ((rc = ll_load_digits(c, d.data(), tests[i].offset))) return bfail(b, rc, "load %u: %s", i, c->msg);
+    b->remaining[i] = tests[i].iterations;
+    // group by schedule
+    BatchGroup *grp = nullptr;
+    for (auto &g : b->groups)
+      if (g.n == c->n && g.sch.path == c->sch.path && g.sch.W == c->sch.W && g.sch.H == c->sch.H &&
+          g.sch.R == c->sch.R && g.sch.rk == c->sch.rk)
+        grp = &g;
+    if (!grp) {
+      b->groups.emplace_back();
+      grp = &b->groups.back();
+      grp->sch = c->sch;
+      grp->n = c->n;
+    }
+    grp->members.push_back(i);
+  }
+  BCHK(b, hipMemcpy(b->d_remaining, b->remaining.data(), 4 * count, hipMemcpyHostToDevice));
+  for (auto &g : b->groups) {
+    const size_t k = g.members.size();
+    if (g.sch.path == 1) {
+      std::vector<PlanArgs> a(k);
+      for (size_t t = 0; t < k; t++) {
+        a[t] = b->tests[g.members[t]]->args;
+        a[t].remaining = b->d_remaining + g.members[t];
+      }
+      if ((rc = balloc(b, &g.d_args, k))) return rc;
+      BCHK(b, hipMemcpy(g.d_args, a.data(), k * sizeof(PlanArgs), hipMemcpyHostToDevice));
+    } else {
+      g.desc.resize(k);
+      for (size_t t = 0; t < k; t++) {
+        g.desc[t] = b->tests[g.members[t]]->rdesc;
+        g.desc[t].remaining = b->d_remaining + g.members[t];
+      }
+      if ((rc = balloc(b, &g.d_desc, k))) return rc;
+    }
+  }
+  return LL_OK;
+}
+
+void ll_batch_destroy(ll_batch *b) {
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->stream2) (void)hipStreamSynchronize(b->stream2);
+  batch_release(b);
+  delete b;
+}
+
+const char *ll_batch_last_error(const ll_batch *b) { return b ? b->msg : "null batch"; }
+
+int ll_batch_step(ll_batch *b, uint32_t iterations, float *step_max_err) {
+  if (!b) return LL_ERR_ARG;
+  if (iterations > kBatchRing) return bfail(b, LL_ERR_ARG, "at most %u iterations per step", kBatchRing);
+  BCHK(b, hipSetDevice(b->device));
+  const uint32_t count = (uint32_t)b->tests.size();
+  std::vector<uint32_t> before = b->remaining;
+  // clear the error rings of the tests
+  for (auto *c : b->tests) {
+    uint32_t *ring = c->sch.path == 0 ? (uint32_t *)c->r_err : c->ring;
+    BCHK(b, launch_fill(ring, 0u, c->ring_size, b->stream));
+  }
+  BCHK(b, hipEventRecord(b->ev[0], b->stream));
+  BCHK(b, hipStreamWaitEvent(b->stream2, b->ev[0], 0));
+  // LDS-resident groups: one launch for the whole step, on the second stream
+  for (auto &g : b->groups) {
+    if (g.sch.path != 0) continue;
+    for (auto &d : g.desc) d.iters = iterations;
+    BCHK(b, hipMemcpyAsync(g.d_desc, g.desc.data(), g.desc.size() * sizeof(ResidentDesc),
+                           hipMemcpyHostToDevice, b->stream2));
+    BCHK(b, launch_resident(g.sch.rk, g.d_desc, (uint32_t)g.desc.size(), b->stream2));
+  }
+  // two-sweep groups: square + carry per iteration, graph-replayed
+  bool any_sweeps = false;
+  for (auto &g : b->groups) any_sweeps |= g.sch.path == 1;
+  if (any_sweeps) {
+    uint32_t left = iterations;
+    if (left >= kGraphIters) {
+      int rc = batch_graph(b);
+      if (rc) return rc;
+      for (; left >= kGraphIters; left -= kGraphIters) BCHK(b, hipGraphLaunch(b->graph, b->stream));
+    }
+    for (; left; left--)
+      for (auto &g : b->groups)
+        if (g.sch.path == 1) {
+          BCHK(b, launch_square(g.sch.sk, g.d_args, (uint32_t)g.members.size(), g.sch.W, b->stream));
+          BCHK(b, launch_carry(g.sch.ck, g.d_args, (uint32_t)g.members.size(), g.sch.H, b->stream));
+        }
+  }
+  BCHK(b, hipEventRecord(b->ev[1], b->stream2));
+  BCHK(b, hipStreamWaitEvent(b->stream, b->ev[1], 0));
+  BCHK(b, hipEventRecord(b->ev[2], b->stream));
+  BCHK(b, hipMemcpyAsync(b->remaining.data(), b->d_remaining, 4 * count, hipMemcpyDeviceToHost, b->stream));
+  std::vector<std::vector<uint32_t>> rings(count);
+  for (uint32_t i = 0; i < count; i++) {
+    ll_ctx *c = b->tests[i];
+    rings[i].resize(c->ring_size);
+    const void *src = c->sch.path == 0 ? (const void *)c->r_err : (const void *)c->ring;
+    BCHK(b, hipMemcpyAsync(rings[i].data(), src, 4 * c->ring_size, hipMemcpyDeviceToHost, b->stream));
+  }
+  BCHK(b, hipStreamSynchronize(b->stream));
+  float ms = 0.f;
+  BCHK(b, hipEventElapsedTime(&ms, b->ev[0], b->ev[2]));
+  b->last_ms = ms;
+  for (uint32_t i = 0; i < count; i++) {
+    ll_ctx *c = b->tests[i];
+    const uint32_t ran = before[i] - b->remaining[i];
+    float m = 0.f;
+    for (uint32_t k = 0; k < ran; k++) {
+      const uint32_t bits = c->sch.path == 0 ? rings[i][k] : rings[i][(c->epoch + k) & (c->ring_size - 1)];
+      float e;
+      memcpy(&e, &bits, 4);
+      m = std::max(m, e);
+    }
+    if (c->sch.path == 1) c->epoch += ran;
+    c->iterations += ran;
+    b->done[i] += ran;
+    b->max_err[i] = std::max(b->max_err[i], m);
+    if (step_max_err) step_max_err[i] = m;
+  }
+  return LL_OK;
+}
+
+double ll_batch_last_step_ms(const ll_batch *b) { return b ? b->last_ms : -1.0; }
+
+int ll_batch_status(const ll_batch *b, uint32_t i, uint64_t *iterations_done, uint32_t *remaining,
+                    float *max_err) {
+  if (!b || i >= b->tests.size()) return LL_ERR_ARG;
+  if (iterations_done) *iterations_done = b->done[i];
+  if (remaining) *remaining = b->remaining[i];
+  if (max_err) *max_err = b->max_err[i];
+  return LL_OK;
+}
+
+int ll_batch_read_digits(ll_batch *b, uint32_t i, int32_t *digits, uint32_t *offset) {
+  if (!b || i >= b->tests.size() || !digits) return LL_ERR_ARG;
+  BCHK(b, hipStreamSynchronize(b->stream));
+  BCHK(b, hipStreamSynchronize(b->stream2));
+  const int rc = ll_read_digits(b->tests[i], digits, offset);
+  if (rc) return bfail(b, rc, "read %u: %s", i, b->tests[i]->msg);
+  return LL_OK;
+}
+
+int ll_batch_groups(const ll_batch *b) { return b ? (int)b->groups.size() : -1; }
 
 }  // extern "C"

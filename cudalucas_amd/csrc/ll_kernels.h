@@ -40,6 +40,7 @@ struct PlanArgs {
   double scale;            // 1 / (4 M) after the squaring round trip
   double base_lo;          // 2^q, q = floor(p / n): the small digit base
   uint32_t rshift;         // log2(R): lines are tiled by R in memory
+  uint32_t *remaining;     // iterations left (batched tests), or null
   uint32_t p, n, W, H, q, b;
 };
 
@@ -49,9 +50,10 @@ struct ResidentDesc {
   const double *iw;   // 2^(-f(j)/n)
   const double *qc;   // cos(2 pi e/M), e <= M/4
   float *err;         // iters floats
-  uint32_t *offset_out;
+  uint32_t *offset;   // shift offset, read at start and written at the end
+  uint32_t *remaining; // iterations left (batched tests), or null
   double scale;       // 1 / (4 M)
-  uint32_t p, m, offset, iters;
+  uint32_t p, m, iters;
 };
 
 struct SquareKernel {
@@ -77,10 +79,16 @@ const CarryKernel *find_carry(uint32_t W, uint32_t R);
 const CarryKernel *carry_table(size_t *count);
 const ResidentKernel *find_resident(uint32_t M);
 
-hipError_t launch_square(const SquareKernel *k, const PlanArgs &A, hipStream_t s);
-hipError_t launch_carry(const CarryKernel *k, const PlanArgs &A, hipStream_t s);
-hipError_t launch_load(const CarryKernel *k, const PlanArgs &A, hipStream_t s);
-hipError_t launch_unload(const CarryKernel *k, const PlanArgs &A, hipStream_t s);
+// Sweep launches over `tests` tests of one shape: d_args is a device array of
+// their PlanArgs, blockIdx.y selects the test.
+hipError_t launch_square(const SquareKernel *k, const PlanArgs *d_args, uint32_t tests, uint32_t W,
+                         hipStream_t s);
+hipError_t launch_carry(const CarryKernel *k, const PlanArgs *d_args, uint32_t tests, uint32_t H,
+                        hipStream_t s);
+hipError_t launch_load(const CarryKernel *k, const PlanArgs *d_args, uint32_t tests, uint32_t H,
+                       hipStream_t s);
+hipError_t launch_unload(const CarryKernel *k, const PlanArgs *d_args, uint32_t tests, uint32_t H,
+                         hipStream_t s);
 hipError_t launch_resident(const ResidentKernel *k, const ResidentDesc *d_descs, uint32_t count,
                            hipStream_t s);
 hipError_t launch_resident_unload(const ResidentDesc *d_descs, uint32_t count, int32_t *digits,

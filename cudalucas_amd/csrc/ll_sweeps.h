@@ -1,0 +1,437 @@
+// Device code of the Lucas-Lehmer iteration s <- s^2 - 2 mod 2^p - 1 on gfx950:
+// the two-sweep kernels and their helpers, instantiated by the ll_*.hip files.
+//
+// One iteration of the reference (CUDALucas.cu:1869-1986: cuFFT INVERSE,
+// square, cuFFT INVERSE, rcb, splice) is rebuilt in two shapes:
+//
+//  * resident_kernel: for N <= 16K words the whole M = N/2-point complex
+//    signal lives in one workgroup's LDS and the kernel loops over many
+//    iterations without touching HBM (batched: one workgroup per exponent).
+//
+//  * two sweeps over HBM for large N, M = W x H.  Complex element
+//    m = a*H + b holds words (x[2m], x[2m+1]); lines b are tiled by R, so the
+//    element lives at ((b / R) W + a) R + (b mod R): a carry workgroup's R
+//    lines are one contiguous block, a column is read in runs of R.
+//      square_sweep : per column pair (k1, W-k1): forward H-point DFT,
+//                     real-pairing squaring, inverse H-point DFT.
+//      carry_sweep  : per block of R lines b: inverse twiddle, inverse W-point
+//                     DFT, unweight/round/-2/balanced carry, carry look-back
+//                     from the previous block, reweight, forward W-point
+//                     DFT, forward twiddle.
+//    Each sweep reads and writes the signal once: 32 N bytes per iteration.
+//
+// Carries between line blocks travel through an epoch-tagged shuttle
+// (64-bit words: iteration epoch in the high half, int32 carry in the low
+// half) written and polled with agent-scope atomics, so no fence or flag is
+// needed.  Blocks take tickets from an atomic counter, so a block only waits
+// on blocks that were admitted before it.  The cyclic wrap (the last block's
+// carries feed block 0) is handled by admitting H/R + 1 workgroups per
+// launch: ticket 0 computes block 0 only to publish its carries; the last
+// ticket recomputes block 0 from the untouched input, applies the wrapped
+// carries and writes it.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ll_fft.h"
+#include "ll_kernels.h"
+
+namespace llamd {
+
+// Upper bound on carry look-back polls (a few hundred ms); reaching it marks
+// the iteration failed (error 3.0) instead of hanging the device.
+constexpr uint32_t kSpinLimit = 1u << 22;
+
+// ------------------------------------------------------------------ helpers
+
+__device__ __forceinline__ cplx twiddle_m(const TwoLevel &t, uint32_t e) {
+  const cplx h = t.hi[e >> t.shift];
+  const cplx l = t.lo[e & t.mask];  // (cos - 1, sin) of a small angle
+  return {h.x + fma(h.x, l.x, -h.y * l.y), h.y + fma(h.x, l.y, h.y * l.x)};
+}
+
+// Balanced digit of value v in base 2^width and the carry out.
+__device__ __forceinline__ int64_t balance(int64_t v, int width, int64_t &carry) {
+  const int64_t half = (int64_t)1 << (width - 1);
+  const int64_t d = ((v + half) & ((half << 1) - 1)) - half;
+  carry = (v - d) >> width;
+  return d;
+}
+
+// The same split in FP64, exact for |v| < 2^53: every value here is an
+// integer and the base 2^width a power of two, so v * 2^-width, its rint and
+// the fma below are all exact.  Ties round to even, so the digit lies in
+// [-2^(width-1), 2^(width-1)].
+__device__ __forceinline__ double balance_f64(double v, double base, double inv_base, double &carry) {
+  carry = rint(v * inv_base);
+  return fma(-carry, base, v);
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+template <int T>
+__device__ __forceinline__ float block_max(float v, float *scratch) {
+  v = wave_max(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) scratch[wave] = v;
+  __syncthreads();
+  float m = 0.f;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int w = 0; w < T / 64; w++) m = fmaxf(m, scratch[w]);
+  }
+  return m;  // valid in thread 0
+}
+
+// Position of the -2 for the iteration that maps offset -> 2*offset mod p:
+// subtract 2^(pos) from digit `digit` (CUDALucas.cu:1882-1886, exact).
+struct Minus2 {
+  uint32_t new_offset;
+  uint32_t digit;
+  int64_t value;
+};
+
+__device__ __forceinline__ Minus2 minus2_position(uint32_t offset, uint32_t p, uint32_t n) {
+  Minus2 m;
+  m.new_offset = (uint32_t)((2ull * offset) % p);
+  uint64_t bit = (uint64_t)m.new_offset + 1;
+  if (bit == p) bit = 0;
+  m.digit = (uint32_t)((bit * n) / p);
+  const uint64_t start = ((uint64_t)m.digit * p + n - 1) / n;
+  m.value = (int64_t)1 << (bit - start);
+  return m;
+}
+
+// Words of the pair (2(aH+b), 2(aH+b)+1) use f = fA[a] + fB[2b+e] (mod n).
+struct WordWeight {
+  double w, iw;
+  int width;
+};
+
+__device__ __forceinline__ WordWeight word_weight(const PlanArgs &A, int fa, double wa, double iwa,
+                                                  int i) {
+  int f = fa + A.fB[i];
+  const bool wrap = f >= (int)A.n;
+  if (wrap) f -= (int)A.n;
+  WordWeight r;
+  r.w = wa * A.wB[i] * (wrap ? 0.5 : 1.0);
+  r.iw = iwa * A.iwB[i] * (wrap ? 2.0 : 1.0);
+  r.width = (int)A.q + (f < (int)A.b ? 1 : 0);
+  return r;
+}
+
+// Real-pairing squaring of the half-length spectrum: with A = Z[k],
+// B = Z[M-k], t = exp(-2 pi i k/M), E2 = A + conj(B), O2 = -i (A - conj(B)),
+//   Z'[k]   = E2^2 + t O2^2 + 2i E2 O2
+//   Z'[M-k] = conj(E2^2 + t O2^2 - 2i E2 O2)
+// which is 4x the half-length spectrum of the squared real signal; the
+// factor 1/4 is applied with the unweighting.
+__device__ __forceinline__ void pair_square(cplx &A, cplx &B, cplx t) {
+  const cplx e2 = {A.x + B.x, A.y - B.y};
+  const cplx o2 = {A.y + B.y, B.x - A.x};
+  const cplx P = csqr(e2);
+  const cplx Q = cmul(t, csqr(o2));
+  const cplx eo = cmul(e2, o2);
+  const cplx S = {-2.0 * eo.y, 2.0 * eo.x};
+  const cplx PQ = cadd(P, Q);
+  A = cadd(PQ, S);
+  B = cconj(csub(PQ, S));
+}
+
+// ------------------------------------------------------------ square sweep
+// W/2 workgroups: workgroup g in [1, W/2) owns the column pair (g, W - g);
+// workgroup 0 owns the two self-paired columns 0 and W/2.  After the forward
+// H-point DFT, element k2 of column k1 holds Z[k1 + W k2]; its partner
+// M - k is (W - k1, H - 1 - k2) for k1 != 0 and (0, H - k2) for k1 = 0.
+template <int H, int T>
+__global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict__ args) {
+  const PlanArgs &A = args[blockIdx.y];
+  if (A.remaining && *A.remaining == 0) return;  // this test has finished
+  constexpr int LP = lpitch(H);
+  __shared__ cplx lds[2 * LP];
+  __shared__ double qc[H / 4 + 1];
+  const int tid = threadIdx.x;
+  const uint32_t W = A.W;
+  const uint32_t g = blockIdx.x;
+  const uint32_t c0 = g;
+  const uint32_t c1 = g == 0 ? W / 2 : W - g;
+  // column c, element b lives at ((b >> rs) W + c) << rs | (b & rmask)
+  const uint32_t rs = A.rshift, rmask = (1u << rs) - 1;
+  const cplx *col0 = A.x + ((size_t)c0 << rs);
+  const cplx *col1 = A.x + ((size_t)c1 << rs);
+  load_quarter<H, T>(qc, A.qcH);
+  for (int i = tid; i < H; i += T) {
+    const size_t off = ((size_t)(i >> rs) * W << rs) | (i & rmask);
+    lds[lpad(i)] = col0[off];
+    lds[LP + lpad(i)] = col1[off];
+  }
+  __syncthreads();
+  const QuarterRoots<H> tw{qc};
+  fft_lds<H, 2, T, false>(lds, tw);
+  if (g != 0) {
+    for (int k2 = tid; k2 < H; k2 += T) {
+      const int k2b = H - 1 - k2;
+      cplx a = lds[lpad(k2)], b = lds[LP + lpad(k2b)];
+      pair_square(a, b, twiddle_m(A.twM, c0 + W * (uint32_t)k2));
+      lds[lpad(k2)] = a;
+      lds[LP + lpad(k2b)] = b;
+    }
+  } else {
+    // column 0: k2 <-> H - k2 ; column W/2: k2 <-> H - 1 - k2
+    for (int k2 = tid; k2 <= H / 2; k2 += T) {
+      const int k2b = (H - k2) % H;
+      cplx a = lds[lpad(k2)], b = lds[lpad(k2b)];
+      pair_square(a, b, twiddle_m(A.twM, W * (uint32_t)k2));
+      lds[lpad(k2)] = a;
+      if (k2b != k2) lds[lpad(k2b)] = b;
+    }
+    for (int k2 = tid; k2 < (H + 1) / 2; k2 += T) {
+      const int k2b = H - 1 - k2;
+      cplx a = lds[LP + lpad(k2)], b = lds[LP + lpad(k2b)];
+      pair_square(a, b, twiddle_m(A.twM, W / 2 + W * (uint32_t)k2));
+      lds[LP + lpad(k2)] = a;
+      if (k2b != k2) lds[LP + lpad(k2b)] = b;
+    }
+  }
+  __syncthreads();
+  fft_lds<H, 2, T, true>(lds, tw);
+  cplx *out0 = A.x + ((size_t)c0 << rs);
+  cplx *out1 = A.x + ((size_t)c1 << rs);
+  for (int i = tid; i < H; i += T) {
+    const size_t off = ((size_t)(i >> rs) * W << rs) | (i & rmask);
+    out0[off] = lds[lpad(i)];
+    out1[off] = lds[LP + lpad(i)];
+  }
+}
+
+// ------------------------------------------------------------- carry sweep
+// H/R workgroups, one per block of R lines, admitted in ticket order.  Each
+// publishes the carries out of its block before waiting for those into it,
+// so every wait is on a block admitted earlier -- except block 0, which
+// waits for the last block's carries (the cyclic wrap 2^p = 1) after
+// publishing its own; no block waits on block 0 finishing, so this cannot
+// deadlock whatever the number of co-resident workgroups.
+template <int W, int R, int T>
+__global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArgs *__restrict__ args) {
+  const PlanArgs &A = args[blockIdx.y];
+  if (A.remaining && *A.remaining == 0) return;  // finished: take no ticket
+  constexpr int LP = lpitch(W);
+  constexpr int CPT = (W + T - 1) / T;  // columns per thread
+  __shared__ cplx lds[R * LP];
+  __shared__ double qc[W / 4 + 1];
+  __shared__ int fb[2 * R];  // weight factors shared by every column
+  __shared__ double wb[2 * R], iwb[2 * R];
+  __shared__ float red[T / 64];
+  __shared__ uint64_t s_ticket;
+  __shared__ Minus2 s_m2;
+  const int tid = threadIdx.x;
+  const uint32_t H = A.H;
+  const uint32_t G = H / R;
+
+  if (tid == 0) {
+    s_ticket = atomicAdd(&A.st->ticket, 1ull);
+    s_m2 = minus2_position(A.st->offset, A.p, A.n);
+  }
+  load_quarter<W, T>(qc, A.qcW);
+  __syncthreads();
+  const uint64_t tk = s_ticket;
+  const uint32_t epoch = (uint32_t)(tk / G);
+  const uint32_t blk = (uint32_t)(tk % G);
+  if (tid < 2 * R) {
+    const uint32_t i = 2 * blk * R + tid;
+    fb[tid] = A.fB[i];
+    wb[tid] = A.wB[i];
+    iwb[tid] = A.iwB[i] * A.scale;
+  }
+  const uint32_t b0 = blk * R;
+  const Minus2 m2 = s_m2;
+  const QuarterRoots<W> tw{qc};
+
+  // load R lines and undo the column twiddle
+  for (int e = tid; e < R * W; e += T) {
+    const int r = e % R, k1 = e / R;
+    cplx v = A.x[(size_t)b0 * W + e];
+    v = cmulc(v, twiddle_m(A.twM, (b0 + r) * (uint32_t)k1));
+    lds[r * LP + lpad(k1)] = v;
+  }
+  __syncthreads();
+  fft_lds<W, R, T, true>(lds, tw);
+
+  // Round, subtract, carry along the 2R consecutive words of each column,
+  // in exact FP64 (balance_f64).  The balanced digits go back into the
+  // column's own LDS slots until the carry-in arrives.
+  const double base_lo = A.base_lo, ibase_lo = 1.0 / base_lo;  // 2^q and 2^-q
+  const double dmod = (double)m2.value;
+  double emax = 0.0;
+  for (int a = tid; a < W; a += T) {
+    const int fa = A.fA[a];
+    const double iwa = A.iwA[a];
+    const uint32_t j0 = 2u * ((uint32_t)a * H + b0);
+    double carry = 0.0;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const cplx v = lds[r * LP + lpad(a)];
+      double d[2];
+#pragma unroll
+      for (int e = 0; e < 2; e++) {
+        const int i = 2 * r + e;
+        int f = fa + fb[i];
+        const bool wrap = f >= (int)A.n;
+        f -= wrap ? (int)A.n : 0;
+        const bool big = f < (int)A.b;
+        const double t = (e ? v.y : v.x) * (iwa * iwb[i] * (wrap ? 2.0 : 1.0));
+        double rt = rint(t);
+        emax = fmax(emax, fabs(t - rt));
+        if (j0 + i == m2.digit) rt -= dmod;
+        d[e] = balance_f64(rt + carry, big ? 2.0 * base_lo : base_lo, big ? 0.5 * ibase_lo : ibase_lo, carry);
+      }
+      lds[r * LP + lpad(a)] = {d[0], d[1]};
+    }
+    const bool fits = fabs(carry) < 2147483648.0;
+    if (!fits) emax = 2.0;  // carry overflow: flagged as a failed iteration
+    const int32_t c32 = fits ? (int32_t)carry : 0;
+    const uint64_t word = ((uint64_t)epoch << 32) | (uint32_t)c32;
+    __hip_atomic_store(&A.shuttle[(size_t)blk * W + a], word, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+
+  // carry-in from the previous block (block 0: the last block, one column
+  // to the left), propagated through the whole chain, then reweight
+  for (int a = tid; a < W; a += T) {
+    const size_t src = blk == 0 ? (size_t)(G - 1) * W + (a + W - 1) % W : (size_t)(blk - 1) * W + a;
+    uint64_t word;
+    uint32_t spins = 0;
+    for (;;) {
+      word = __hip_atomic_load(&A.shuttle[src], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if ((uint32_t)(word >> 32) == epoch) break;
+      if (++spins > kSpinLimit) {  // never hang: flag the iteration as failed
+        emax = 3.0;
+        word = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    // What is left after one word is about sqrt(N)/12 whatever the digit
+    // size, which exceeds half a digit at low bits/word, so the carry runs
+    // through the whole chain; anything left after 2R words (0 in
+    // practice) stays in the last word.
+    double cc = (double)(int32_t)(uint32_t)word;
+    const int fa = A.fA[a];
+    const double wa = A.wA[a];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const cplx dv = lds[r * LP + lpad(a)];
+      double x[2];
+#pragma unroll
+      for (int e = 0; e < 2; e++) {
+        const int i = 2 * r + e;
+        int f = fa + fb[i];
+        const bool wrap = f >= (int)A.n;
+        f -= wrap ? (int)A.n : 0;
+        const bool big = f < (int)A.b;
+        const double base = big ? 2.0 * base_lo : base_lo;
+        double d = balance_f64((e ? dv.y : dv.x) + cc, base, big ? 0.5 * ibase_lo : ibase_lo, cc);
+        if (i == 2 * R - 1) d = fma(cc, base, d);
+        x[e] = d * (wa * wb[i] * (wrap ? 0.5 : 1.0));
+      }
+      lds[r * LP + lpad(a)] = {x[0], x[1]};
+    }
+  }
+  __syncthreads();
+  fft_lds<W, R, T, false>(lds, tw);
+  for (int e = tid; e < R * W; e += T) {
+    const int r = e % R, k1 = e / R;
+    const cplx v = lds[r * LP + lpad(k1)];
+    A.x[(size_t)b0 * W + e] = cmul(v, twiddle_m(A.twM, (b0 + r) * (uint32_t)k1));
+  }
+  const float m = block_max<T>((float)emax, red);
+  if (tid == 0) {
+    atomicMax(&A.err[epoch & A.err_mask], __float_as_uint(m));
+    const uint32_t old = atomicAdd(&A.st->done, 1u);
+    if (old == G - 1) {
+      A.st->done = 0;
+      A.st->offset = m2.new_offset;
+      if (A.remaining) *A.remaining -= 1;
+    }
+  }
+}
+
+// ------------------------------------------------------ load / unload (HBM)
+// load: balanced digits (natural order) -> weighted, line-transformed,
+// twiddled signal, the state the square sweep expects.
+template <int W, int R, int T>
+__global__ __launch_bounds__(T) void load_lines(const PlanArgs *__restrict__ args) {
+  const PlanArgs &A = args[blockIdx.y];
+  constexpr int LP = lpitch(W);
+  __shared__ cplx lds[R * LP];
+  __shared__ double qc[W / 4 + 1];
+  load_quarter<W, T>(qc, A.qcW);
+  const QuarterRoots<W> tw{qc};
+  const int tid = threadIdx.x;
+  const uint32_t H = A.H;
+  const uint32_t b0 = blockIdx.x * R;
+  for (int a = tid; a < W; a += T) {
+    const int fa = A.fA[a];
+    const double wa = A.wA[a], iwa = A.iwA[a];
+    for (int r = 0; r < R; r++) {
+      const uint32_t j = 2u * ((uint32_t)a * H + b0 + r);
+      const WordWeight we = word_weight(A, fa, wa, iwa, 2 * (b0 + r));
+      const WordWeight wo = word_weight(A, fa, wa, iwa, 2 * (b0 + r) + 1);
+      lds[r * LP + lpad(a)] = {(double)A.digits[j] * we.w, (double)A.digits[j + 1] * wo.w};
+    }
+  }
+  __syncthreads();
+  fft_lds<W, R, T, false>(lds, tw);
+  for (int e = tid; e < R * W; e += T) {
+    const int r = e % R, k1 = e / R;
+    const cplx v = lds[r * LP + lpad(k1)];
+    A.x[(size_t)b0 * W + e] = cmul(v, twiddle_m(A.twM, (b0 + r) * (uint32_t)k1));
+  }
+}
+
+// unload: inverse of load; digits rounded, rounding error max into err[0].
+template <int W, int R, int T>
+__global__ __launch_bounds__(T) void unload_lines(const PlanArgs *__restrict__ args) {
+  const PlanArgs &A = args[blockIdx.y];
+  constexpr int LP = lpitch(W);
+  __shared__ cplx lds[R * LP];
+  __shared__ double qc[W / 4 + 1];
+  __shared__ float red[T / 64];
+  load_quarter<W, T>(qc, A.qcW);
+  const QuarterRoots<W> tw{qc};
+  const int tid = threadIdx.x;
+  const uint32_t H = A.H;
+  const uint32_t b0 = blockIdx.x * R;
+  for (int e = tid; e < R * W; e += T) {
+    const int r = e % R, k1 = e / R;
+    cplx v = A.x[(size_t)b0 * W + e];
+    lds[r * LP + lpad(k1)] = cmulc(v, twiddle_m(A.twM, (b0 + r) * (uint32_t)k1));
+  }
+  __syncthreads();
+  fft_lds<W, R, T, true>(lds, tw);
+  float emax = 0.f;
+  const double scale = 1.0 / (double)W;
+  for (int a = tid; a < W; a += T) {
+    const int fa = A.fA[a];
+    const double wa = A.wA[a], iwa = A.iwA[a];
+    for (int r = 0; r < R; r++) {
+      const uint32_t j = 2u * ((uint32_t)a * H + b0 + r);
+      const cplx v = lds[r * LP + lpad(a)];
+      const WordWeight we = word_weight(A, fa, wa, iwa, 2 * (b0 + r));
+      const WordWeight wo = word_weight(A, fa, wa, iwa, 2 * (b0 + r) + 1);
+      const double t0 = v.x * we.iw * scale, t1 = v.y * wo.iw * scale;
+      const double r0 = rint(t0), r1 = rint(t1);
+      emax = fmaxf(emax, fmaxf((float)fabs(t0 - r0), (float)fabs(t1 - r1)));
+      A.digits[j] = (int32_t)r0;
+      A.digits[j + 1] = (int32_t)r1;
+    }
+  }
+  const float m = block_max<T>(emax, red);
+  if (tid == 0) atomicMax(&A.err[0], __float_as_uint(m));
+}
+
+}  // namespace llamd

@@ -112,6 +112,31 @@ int ll_bench_copy(int32_t device, uint64_t bytes, uint32_t reps, float *ms);
  * kind 0 clears. */
 int ll_debug_inject(ll_ctx *ctx, uint32_t kind, uint64_t after, float value);
 
+/* ---- batched engine (no reference counterpart: CUDALucas runs one test
+ * per process) --------------------------------------------------------- */
+typedef struct ll_batch ll_batch;
+typedef struct ll_batch_test {
+  uint32_t p, n;       /* exponent and FFT length (words) */
+  uint32_t offset;     /* shift offset of s_0 = 4 */
+  uint32_t iterations; /* iterations to run (p - 2 for a full test) */
+} ll_batch_test;
+/* Builds every test's tables and state (s_0 = 4 at its offset), groups the
+ * tests by FFT length and schedule. flags: LL_OPT_FORCE_SWEEPS. */
+int ll_batch_create(int32_t device, uint32_t count, const ll_batch_test *tests, uint32_t flags,
+                    ll_batch **out);
+void ll_batch_destroy(ll_batch *batch);
+/* Advances every unfinished test by up to `iterations` (<= 1024) LL
+ * iterations; step_max_err (may be NULL) receives, per test, the maximum
+ * roundoff of the iterations it ran in this step. */
+int ll_batch_step(ll_batch *batch, uint32_t iterations, float *step_max_err);
+int ll_batch_status(const ll_batch *batch, uint32_t index, uint64_t *iterations_done,
+                    uint32_t *remaining, float *max_err);
+/* Current digits of test `index` (as ll_read_digits). */
+int ll_batch_read_digits(ll_batch *batch, uint32_t index, int32_t *digits, uint32_t *offset);
+double ll_batch_last_step_ms(const ll_batch *batch);
+int ll_batch_groups(const ll_batch *batch);
+const char *ll_batch_last_error(const ll_batch *batch);
+
 /* ---- host codec (CUDALucas.cu:1280-1342, 1465-1614) ------------------ */
 int ll_digit_width(uint32_t p, uint32_t n, uint32_t j);
 /* unpack_bits (CUDALucas.cu:1285-1311) */

@@ -209,3 +209,23 @@ def test_c3_smooth_lengths_vs_gmp_fixture(n, gmp_fixtures):
     assert sha(w) == fx["sha256"]
     print("C3 n=%d p=%d bits/word=%.3f max roundoff %.4f" % (n, p, p / n, m))
     assert m < 0.45
+
+
+SHORT_LINE_LENGTHS_K = [18, 20, 32, 36, 42, 48, 50, 56, 60, 64, 70, 80, 84, 96, 112, 120, 126, 144, 160,
+                        180, 224]
+
+
+@pytest.mark.parametrize("nk", SHORT_LINE_LENGTHS_K)
+def test_short_line_lengths_vs_gmp(nk):
+    """Lengths whose line FFT carries the 3/5/7 factors (W = 72..448)."""
+    n = nk * 1024
+    p = W.next_prime(int(n * 18.2))
+    iters = 300
+    announce("short-line length n=%d p=%d" % (n, p))
+    with L.LLContext(p, n) as c:
+        info = c.info()
+        c.load_initial(p // 5)
+        m = c.iterate(iters)
+        w, _ = c.residue()
+    assert (w == O.gmp_ll(p, iters)).all(), info
+    assert m < ROUNDOFF_LIMIT
