@@ -68,6 +68,52 @@ def cpu_baseline(p, n, budget_s):
     }
 
 
+def batched_leg(steps=4, step_iters=512, plain_iters=200):
+    """C4 workload: 256 seeded exponents log-uniform in [1e5, 5e6] at their
+    chosen FFT lengths.  Exponents/hour, with finished tests replaced by new
+    ones of the same mix: batched = 3600 * 256 / (t_step * mean(p - 2));
+    plain (one exponent at a time on the same GPU) =
+    3600 / mean((p - 2) * t_iter(p))."""
+    import numpy as np
+
+    import cudalucas_amd as L
+    from cudalucas_amd import workloads as W
+
+    ps = W.log_uniform_primes(W.C4_SEED, W.C4_COUNT, *W.C4_RANGE)
+    lens = [L.choose_fft_length(p)[0] for p in ps]
+    tests = [(p, n, p % 1000, p - 2) for p, n in zip(ps, lens)]
+    with L.LLBatch(tests) as b:
+        b.step(64)
+        total_ms, worst = 0.0, 0.0
+        for _ in range(steps):
+            e = b.step(step_iters)
+            total_ms += b.last_step_ms()
+            worst = max(worst, float(e.max()))
+        groups = b.groups()
+    t_step = total_ms * 1e-3 / (steps * step_iters)
+    mean_iters = float(np.mean([p - 2 for p in ps]))
+    batched = 3600.0 * len(ps) / (t_step * mean_iters)
+    per = []
+    for p, n in zip(ps, lens):
+        with L.LLContext(p, n) as c:
+            c.load_initial(p % 1000)
+            c.iterate(16)
+            c.iterate(plain_iters)
+            per.append((p - 2) * c.last_iterate_ms() * 1e-3 / plain_iters)
+    plain = 3600.0 / float(np.mean(per))
+    return {
+        "workload": "256 seeded primes log-uniform in [1e5, 5e6], FFT lengths %dK-%dK (%d schedule groups)"
+        % (min(lens) // 1024, max(lens) // 1024, groups),
+        "exps_per_hour_batched": round(batched, 1),
+        "exps_per_hour_plain": round(plain, 1),
+        "speedup": round(batched / plain, 2),
+        "batch_iteration_ms": round(t_step * 1e3, 4),
+        "sample": "%d timed batch iterations after 64 warm-up; plain: %d iterations per exponent, all 256"
+        % (steps * step_iters, plain_iters),
+        "max_roundoff": round(worst, 4),
+    }
+
+
 def load_traffic(name):
     path = os.path.join(HERE, "profiles", name)
     if not os.path.exists(path):
@@ -86,6 +132,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-only", action="store_true",
                     help="only the 64 MiB streaming-copy calibration kernel")
+    ap.add_argument("--no-batched", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; the LL path of one exponent runs on a "
@@ -136,7 +183,7 @@ def main():
         "copy_kernel_gbs": round(copy_gbs, 1),
     }
     out = {
-        "metric": "ms/iter at 4M FFT (p~77M)",
+        "metric": "ms/iter at 4M FFT (p~77M) + batched exps/hour vs plain; HBM GB/s",
         "value": round(ms_per_step, 5),
         "unit": "ms/iter",
         "n_gpus": 1,
@@ -165,6 +212,8 @@ def main():
         "res64_iteration": done_iters,
         "roofline": roofline,
     }
+    if not args.no_batched:
+        out["batched"] = batched_leg()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, n, args.cpu_budget)
     print(json.dumps(out))
