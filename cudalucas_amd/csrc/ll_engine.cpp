@@ -616,6 +616,7 @@ struct ll_batch {
   std::vector<uint32_t> remaining;   // host copy after the last step
   std::vector<uint64_t> done;        // iterations run per test
   std::vector<float> max_err;        // running maximum per test
+  std::vector<std::vector<float>> step_errs;  // per-iteration errors of the last step
   hipGraphExec_t graph = nullptr;
   double last_ms = 0.0;
   char msg[256] = {0};
@@ -806,14 +807,17 @@ int ll_batch_step(ll_batch *b, uint32_t iterations, float *step_max_err) {
   float ms = 0.f;
   BCHK(b, hipEventElapsedTime(&ms, b->ev[0], b->ev[2]));
   b->last_ms = ms;
+  b->step_errs.resize(count);
   for (uint32_t i = 0; i < count; i++) {
     ll_ctx *c = b->tests[i];
     const uint32_t ran = before[i] - b->remaining[i];
     float m = 0.f;
+    b->step_errs[i].resize(ran);
     for (uint32_t k = 0; k < ran; k++) {
       const uint32_t bits = c->sch.path == 0 ? rings[i][k] : rings[i][(c->epoch + k) & (c->ring_size - 1)];
       float e;
       memcpy(&e, &bits, 4);
+      b->step_errs[i][k] = e;
       m = std::max(m, e);
     }
     if (c->sch.path == 1) c->epoch += ran;
@@ -846,5 +850,33 @@ int ll_batch_read_digits(ll_batch *b, uint32_t i, int32_t *digits, uint32_t *off
 }
 
 int ll_batch_groups(const ll_batch *b) { return b ? (int)b->groups.size() : -1; }
+
+int ll_batch_step_errors(const ll_batch *b, uint32_t i, float *errs, uint32_t capacity) {
+  if (!b || i >= b->tests.size()) return LL_ERR_ARG;
+  if (i >= b->step_errs.size()) return 0;
+  const auto &e = b->step_errs[i];
+  if (errs && capacity < e.size()) return LL_ERR_ARG;
+  if (errs) memcpy(errs, e.data(), e.size() * sizeof(float));
+  return (int)e.size();
+}
+
+int ll_fft_schedule(uint32_t n, const ll_options *opts, ll_info *info) {
+  Schedule s;
+  if (!info) return LL_ERR_ARG;
+  if (!pick_schedule(n, opts, &s)) return LL_ERR_UNSUPPORTED;
+  memset(info, 0, sizeof(*info));
+  info->n = n;
+  info->path = s.path;
+  if (s.path == 0) {
+    info->width = n / 2;
+    info->height = 1;
+    info->lines = 1;
+  } else {
+    info->width = s.W;
+    info->height = s.H;
+    info->lines = s.R;
+  }
+  return LL_OK;
+}
 
 }  // extern "C"

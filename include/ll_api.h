@@ -134,8 +134,72 @@ int ll_batch_status(const ll_batch *batch, uint32_t index, uint64_t *iterations_
 /* Current digits of test `index` (as ll_read_digits). */
 int ll_batch_read_digits(ll_batch *batch, uint32_t index, int32_t *digits, uint32_t *offset);
 double ll_batch_last_step_ms(const ll_batch *batch);
+/* Per-iteration roundoff of test `index` over the last ll_batch_step (one
+ * value per iteration it ran, in order).  Returns the count, or LL_ERR_ARG
+ * if `capacity` is too small; errs may be NULL to query the count. */
+int ll_batch_step_errors(const ll_batch *batch, uint32_t index, float *errs, uint32_t capacity);
 int ll_batch_groups(const ll_batch *batch);
 const char *ll_batch_last_error(const ll_batch *batch);
+
+/* ---- roundoff surrogate for FFT-length selection (no reference
+ * counterpart: the reference picks lengths from fixed power laws,
+ * CUDALucas.cu:1027-1068, and corrects with roundoff retries,
+ * CUDALucas.cu:2600-2640).  An FP32 MLP, tanh hidden layers and one linear
+ * output, trained with mean squared error and Adam, all on the GPU.
+ * Parameters are one vector: per layer l, W_l [sizes[l+1] x sizes[l]]
+ * row-major then b_l [sizes[l+1]]. Widths <= 128; sizes[layers] must be 1. */
+typedef struct ll_mlp ll_mlp;
+int ll_mlp_create(int32_t device, uint32_t layers, const uint32_t *sizes, ll_mlp **out);
+void ll_mlp_destroy(ll_mlp *mlp);
+const char *ll_mlp_last_error(const ll_mlp *mlp);
+uint32_t ll_mlp_param_count(const ll_mlp *mlp);
+/* Replaces the parameters and clears the Adam moments. */
+int ll_mlp_set_params(ll_mlp *mlp, const float *params);
+int ll_mlp_get_params(ll_mlp *mlp, float *params);
+/* y[batch] = f(x[batch][sizes[0]]) */
+int ll_mlp_forward(ll_mlp *mlp, const float *x, uint32_t batch, float *y);
+/* Loss = mean (f(x) - t)^2 over the batch; grad (may be NULL) receives
+ * dLoss/dparams, loss (may be NULL) the loss. Parameters are unchanged. */
+int ll_mlp_gradient(ll_mlp *mlp, const float *x, const float *t, uint32_t batch, float *grad,
+                    float *loss);
+/* `epochs` passes of minibatch Adam (beta 0.9 / 0.999, eps 1e-8) over
+ * `count` samples in seeded shuffled order; a trailing partial batch is
+ * dropped. epoch_loss (may be NULL) receives each epoch's mean batch loss. */
+int ll_mlp_train(ll_mlp *mlp, const float *x, const float *t, uint32_t count, uint32_t batch,
+                 uint32_t epochs, float lr, uint64_t seed, float *epoch_loss);
+
+/* The selector: features of (p, n) -- log2 p, log2 n, bits per word,
+ * exponents of 3, 5 and 7 in n, 1 if n runs LDS-resident -- feed the MLP,
+ * whose output is log2 of the maximum roundoff over a window of iterations
+ * (after normalisation; both affine maps are stored in the model file).
+ * The prediction is extrapolated to the p - 2 iterations of a test by the
+ * sqrt(ln(#outputs)) growth of a Gaussian maximum and raised by the model's
+ * calibrated margin. */
+#define LL_SELECTOR_FEATURES 7
+typedef struct ll_selector ll_selector;
+/* Writes LL_SELECTOR_FEATURES raw features; returns their count or < 0. */
+int ll_selector_features(uint32_t p, uint32_t n, float *features);
+/* Lengths the selector considers for p: supported multiples of 1K words in
+ * [0.62, 1.25] x the reference policy's length. Returns the count (ns may
+ * be NULL to query it). */
+int ll_selector_candidates(uint32_t p, uint32_t *ns, uint32_t capacity);
+/* Loads a text model ("cudalucas_amd roundoff-mlp 1") onto `device`. */
+int ll_selector_load(int32_t device, const char *path, ll_selector **out);
+void ll_selector_destroy(ll_selector *sel);
+const char *ll_selector_last_error(const ll_selector *sel);
+/* Measured ms/iteration per length (the fft bench table); with a table the
+ * selector takes the fastest acceptable length, without one the shortest. */
+int ll_selector_set_times(ll_selector *sel, uint32_t count, const uint32_t *ns, const float *ms);
+/* log2 of the predicted full-test maximum roundoff at each length. */
+int ll_selector_predict(ll_selector *sel, uint32_t p, uint32_t count, const uint32_t *ns,
+                        float *log2_err);
+/* Replaces choose_fft_length (CUDALucas.cu:1027-1068) when a model is
+ * loaded.  above = 0: the first choice, never longer than the reference
+ * policy's length (which is returned if no shorter one is predicted safe).
+ * above = n: the fallback after length n failed the round-off test
+ * (CUDALucas.cu:2589-2648): the best acceptable length longer than n.
+ * pred_err (may be NULL) receives the predicted full-test maximum. */
+int ll_selector_choose(ll_selector *sel, uint32_t p, uint32_t above, float *pred_err);
 
 /* ---- host codec (CUDALucas.cu:1280-1342, 1465-1614) ------------------ */
 int ll_digit_width(uint32_t p, uint32_t n, uint32_t j);
@@ -193,6 +257,11 @@ int ll_fft_length_count(void);
 int ll_fft_length_at(int index); /* words, from the reference's default list */
 /* 1 if this build has kernels for length n */
 int ll_fft_supported(uint32_t n);
+/* The schedule ll_create would use for length n under opts (may be NULL):
+ * fills info->n, path, width, height, lines (resident: width = n/2,
+ * height = lines = 1). LL_ERR_UNSUPPORTED if there is none. The analog of
+ * the reference's thread/plan tables (CUDALucas.cu:812-945). */
+int ll_fft_schedule(uint32_t n, const ll_options *opts, ll_info *info);
 /* choose_fft_length (CUDALucas.cu:1027-1068): *index < 0 selects from the
  * list by the lb/ub power laws; *index >= count is a manual length.  Lengths
  * without kernels in this build are skipped upward. Returns n or < 0. */
