@@ -133,6 +133,30 @@ def lib():
     L.ll_fft_bounds.argtypes = [u32, ctypes.POINTER(i32), ctypes.POINTER(i32)]
     L.ll_fft_bounds.restype = None
     L.ll_fft_from_str.argtypes = [ctypes.c_char_p]
+    L.ll_fft_schedule.argtypes = [u32, ctypes.POINTER(_Options), ctypes.POINTER(_Info)]
+    L.ll_batch_step_errors.argtypes = [P, u32, P, u32]
+    L.ll_mlp_create.argtypes = [i32, u32, P, ctypes.POINTER(P)]
+    L.ll_mlp_destroy.argtypes = [P]
+    L.ll_mlp_destroy.restype = None
+    L.ll_mlp_last_error.argtypes = [P]
+    L.ll_mlp_last_error.restype = ctypes.c_char_p
+    L.ll_mlp_param_count.argtypes = [P]
+    L.ll_mlp_param_count.restype = u32
+    L.ll_mlp_set_params.argtypes = [P, P]
+    L.ll_mlp_get_params.argtypes = [P, P]
+    L.ll_mlp_forward.argtypes = [P, P, u32, P]
+    L.ll_mlp_gradient.argtypes = [P, P, P, u32, P, P]
+    L.ll_mlp_train.argtypes = [P, P, P, u32, u32, u32, f32, u64, P]
+    L.ll_selector_features.argtypes = [u32, u32, P]
+    L.ll_selector_candidates.argtypes = [u32, P, u32]
+    L.ll_selector_load.argtypes = [i32, ctypes.c_char_p, ctypes.POINTER(P)]
+    L.ll_selector_destroy.argtypes = [P]
+    L.ll_selector_destroy.restype = None
+    L.ll_selector_last_error.argtypes = [P]
+    L.ll_selector_last_error.restype = ctypes.c_char_p
+    L.ll_selector_set_times.argtypes = [P, u32, P, P]
+    L.ll_selector_predict.argtypes = [P, u32, u32, P, P]
+    L.ll_selector_choose.argtypes = [P, u32, u32, ctypes.POINTER(f32)]
     _lib = L
     return L
 
@@ -221,6 +245,15 @@ def fft_bounds(p):
     lb, ub = ctypes.c_int32(), ctypes.c_int32()
     lib().ll_fft_bounds(p, ctypes.byref(lb), ctypes.byref(ub))
     return lb.value, ub.value
+
+
+def fft_schedule(n, width=0, height=0, lines=0, flags=0):
+    """(path, W, H, R) ll_create would use for length n, or None."""
+    o = _Options(0, (ctypes.c_int32 * 2)(0, 0), width, height, lines, flags)
+    info = _Info()
+    if lib().ll_fft_schedule(n, ctypes.byref(o), ctypes.byref(info)):
+        return None
+    return ("resident" if info.path == 0 else "sweeps", info.width, info.height, info.lines)
 
 
 def fft_from_str(s):
@@ -389,12 +422,87 @@ class LLBatch:
         w = residue_words(p, n, d, off.value)
         return w, int(w[0]) | (int(w[1]) << 32)
 
+    def step_errors(self, i):
+        """Per-iteration roundoff of test i over the last step."""
+        k = lib().ll_batch_step_errors(self._h, i, None, 0)
+        self._check(min(k, 0))
+        out = np.zeros(k, dtype=np.float32)
+        if k:
+            self._check(min(lib().ll_batch_step_errors(self._h, i, _ptr(out), k), 0))
+        return out
+
     def run(self, max_step=MAX_STEP):
         """Steps until every test has finished; returns per-test max roundoff."""
         worst = np.zeros(len(self.tests), dtype=np.float32)
         while any(self.remaining()):
             worst = np.maximum(worst, self.step(max_step))
         return worst
+
+
+class MLP:
+    """FP32 MLP on the GPU (ll_mlp_*): tanh hidden layers, linear scalar
+    output, mean-squared-error loss, Adam.  sizes = [inputs, hidden..., 1]."""
+
+    def __init__(self, sizes, device=0):
+        self.sizes = [int(x) for x in sizes]
+        arr = np.asarray(self.sizes, dtype=np.uint32)
+        h = ctypes.c_void_p()
+        rc = lib().ll_mlp_create(device, len(self.sizes) - 1, _ptr(arr), ctypes.byref(h))
+        self._h = h.value
+        if rc:
+            msg = lib().ll_mlp_last_error(self._h).decode() if self._h else "ll_mlp_create"
+            self.close()
+            raise LLError(rc, msg)
+        self.count = lib().ll_mlp_param_count(self._h)
+
+    def _check(self, rc):
+        if rc:
+            raise LLError(rc, lib().ll_mlp_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ll_mlp_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        self.close()
+
+    def get_params(self):
+        out = np.zeros(self.count, dtype=np.float32)
+        self._check(lib().ll_mlp_get_params(self._h, _ptr(out)))
+        return out
+
+    def set_params(self, params):
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        assert params.shape == (self.count,)
+        self._check(lib().ll_mlp_set_params(self._h, _ptr(params)))
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros(len(x), dtype=np.float32)
+        self._check(lib().ll_mlp_forward(self._h, _ptr(x), len(x), _ptr(y)))
+        return y
+
+    def gradient(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.zeros(self.count, dtype=np.float32)
+        loss = ctypes.c_float()
+        self._check(lib().ll_mlp_gradient(self._h, _ptr(x), _ptr(t), len(t), _ptr(g), ctypes.byref(loss)))
+        return loss.value, g
+
+    def train(self, x, t, batch, epochs, lr, seed=1):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        losses = np.zeros(epochs, dtype=np.float32)
+        self._check(lib().ll_mlp_train(self._h, _ptr(x), _ptr(t), len(t), batch, epochs, lr, seed, _ptr(losses)))
+        return losses
 
 
 def bench_copy(nbytes, reps=20, device=0):

@@ -45,7 +45,25 @@ struct Config {
   int64_t stop_after = -1;  // iterations before a checkpoint-and-exit
   uint32_t force_width = 0, force_height = 0, force_lines = 0;
   bool force_fft = false;  // take -f as given, without the lb/ub clamp
+  bool info = false;       // -info
+  int cufftbench[3] = {0, 0, 0};   // -cufftbench start end passes
+  int threadbench[4] = {0, 0, 0, 0};  // -threadbench start end passes mode
+  std::string selector_model, selector_times;  // learned FFT-length selection
 };
+
+// The learned length selector (--selector), when loaded.
+static ll_selector *g_sel = nullptr;
+
+// A length-selection step: the learned selector when one is loaded, else
+// the reference policy's list index (choose_fft_length, CUDALucas.cu:1027).
+static int next_length(uint32_t q, int &idx, int n_failed) {
+  if (g_sel) {
+    const int n = ll_selector_choose(g_sel, q, (uint32_t)n_failed, nullptr);
+    if (n > 0) return n;
+  }
+  idx++;
+  return ll_choose_fft_length(q, &idx);
+}
 
 static volatile sig_atomic_t g_quit = 0;
 
@@ -246,11 +264,26 @@ static int check(uint32_t q, const Config &cfg, const char *aid) {
   int idx = cfg.fftlen > 0 ? cfg.fftlen : (which >= 0 ? (int)good.meta.n : -1);
   int n = cfg.force_fft && cfg.fftlen > 0 && ll_fft_supported((uint32_t)cfg.fftlen) ? cfg.fftlen
                                                                                   : ll_choose_fft_length(q, &idx);
+  if (g_sel && cfg.fftlen <= 0) {
+    if (which >= 0 && ll_fft_supported(good.meta.n)) {
+      n = (int)good.meta.n;  // keep the length the checkpoint was written at
+    } else if (which < 0) {
+      float pred = 0.f;
+      const int ns = ll_selector_choose(g_sel, q, 0, &pred);
+      if (ns > 0) {
+        printf("Learned length selection: %dK (predicted max roundoff %.3f; length policy %dK)\n", ns / 1024,
+               pred, n / 1024);
+        n = ns;
+      } else {
+        fprintf(stderr, "selector: %s\n", ll_selector_last_error(g_sel));
+      }
+    }
+  }
   if (n <= 0) {
     fprintf(stderr, "No FFT length available for exponent %u\n", q);
     return 2;
   }
-  const int base_idx = idx;
+  const int base_idx = idx, base_n = n;
   int32_t ub_lb[2];
   ll_fft_bounds(q, &ub_lb[0], &ub_lb[1]);
   Engine eng;
@@ -283,8 +316,7 @@ static int check(uint32_t q, const Config &cfg, const char *aid) {
       const int r = round_off_test(eng, last < 1000 ? last : 1000);
       if (r < 0) return 2;
       if (r == 0) break;
-      idx++;
-      n = ll_choose_fft_length(q, &idx);
+      n = next_length(q, idx, n);
       if (n <= 0 || eng.open(cfg, q, (uint32_t)n) || load_snapshot(eng, good)) return 2;
     }
     // the test ran real iterations: continue from where they left off
@@ -351,8 +383,7 @@ static int check(uint32_t q, const Config &cfg, const char *aid) {
         return 2;
       } else {
         printf("The error persists.\nTrying a larger fft until the next checkpoint.\n\n");
-        idx++;
-        n = ll_choose_fft_length(q, &idx);
+        n = next_length(q, idx, n);
         if (n <= 0) return 2;
         fft_reset = true;
       }
@@ -422,8 +453,8 @@ static int check(uint32_t q, const Config &cfg, const char *aid) {
         fft_reset = false;
         retry = true;
         idx = base_idx;
-        n = ll_choose_fft_length(q, &idx);
-        if (n <= 0 || eng.open(cfg, q, (uint32_t)n) || load_snapshot(eng, good)) return 2;
+        n = base_n;
+        if (eng.open(cfg, q, (uint32_t)n) || load_snapshot(eng, good)) return 2;
       } else if (!retry) {
         printf("Looks like the error went away, continuing.\n\n");
         retry = true;
@@ -620,6 +651,239 @@ static void clear_assignment(const std::string &file, uint32_t q) {
   fclose(f);
 }
 
+// ------------------------------------------------------- tuning files
+// "<device name> fft.txt" and "<device name> schedules.txt" in the working
+// directory, as the reference keeps "<device name> fft.txt" and
+// "<device name> threads.txt" (CUDALucas.cu:830, 975).
+static std::string device_file(int device, const char *suffix) {
+  ll_device_info d;
+  if (ll_get_device_info(device < 0 ? 0 : device, &d) != LL_OK) return "";
+  return std::string(d.name) + " " + suffix;
+}
+
+// init_device with show_prop (CUDALucas.cu:737-810)
+static void print_device_info(int device) {
+  ll_device_info d;
+  if (ll_get_device_info(device, &d) != LL_OK) {
+    printf("device %d not available\n\n", device);
+    return;
+  }
+  printf("%s %d-bit build\n\n", kProgram, (int)(sizeof(void *) * 8));
+  printf("HIP runtime version       %d\n", d.runtime_version);
+  printf("HIP driver version        %d\n\n", d.driver_version);
+  printf("---------------- DEVICE %d ----------------\n", device);
+  printf("Device Name               %s\n", d.name);
+  printf("Architecture              %s\n", d.arch);
+  printf("clockRate (MHz)           %d\n", d.clock_mhz);
+  printf("memClockRate (MHz)        %d\n", d.mem_clock_mhz);
+  printf("memBusWidth (bits)        %d\n", d.mem_bus_bits);
+  printf("totalGlobalMem            %llu\n", (unsigned long long)d.global_mem);
+  printf("l2CacheSize               %d\n", d.l2_bytes);
+  printf("sharedMemPerBlock         %u\n", d.lds_per_block);
+  printf("wavefrontSize             %d\n", d.wavefront);
+  printf("maxThreadsPerBlock        %d\n", d.max_threads_per_block);
+  printf("multiProcessorCount       %d\n", d.compute_units);
+  printf("pciDeviceID               %d\n", d.pci_device);
+  printf("pciBusID                  %d\n\n", d.pci_bus);
+}
+
+// init_ffts (CUDALucas.cu:947-1025): the lengths listed in fft.txt, with
+// the defaults below its first and above its last entry.
+static void load_fft_file(const std::string &path) {
+  FILE *f = path.empty() ? nullptr : fopen(path.c_str(), "r");
+  if (!f) return;
+  std::vector<uint32_t> file;
+  char buf[256];
+  while (fgets(buf, sizeof buf, f)) {
+    const int k = atoi(buf);
+    if (k > 0 && (file.empty() || (uint32_t)k * 1024u > file.back())) file.push_back((uint32_t)k * 1024u);
+  }
+  fclose(f);
+  if (file.empty()) return;
+  ll_fft_length_list_set(0, nullptr);
+  std::vector<uint32_t> merged;
+  const int count = ll_fft_length_count();
+  for (int i = 0; i < count; i++)
+    if ((uint32_t)ll_fft_length_at(i) < file.front()) merged.push_back((uint32_t)ll_fft_length_at(i));
+  merged.insert(merged.end(), file.begin(), file.end());
+  for (int i = 0; i < count; i++)
+    if ((uint32_t)ll_fft_length_at(i) > file.back()) merged.push_back((uint32_t)ll_fft_length_at(i));
+  ll_fft_length_list_set((uint32_t)merged.size(), merged.data());
+}
+
+// init_threads reading threads.txt (CUDALucas.cu:812-945), for schedules:
+// lines "K path W H R ms", a later line for the same length wins.
+static void load_schedule_file(const std::string &path) {
+  FILE *f = path.empty() ? nullptr : fopen(path.c_str(), "r");
+  if (!f) return;
+  std::vector<ll_schedule> t;
+  char buf[256];
+  while (fgets(buf, sizeof buf, f)) {
+    unsigned k, path_, w, h, r;
+    float ms;
+    if (sscanf(buf, "%u %u %u %u %u %f", &k, &path_, &w, &h, &r, &ms) != 6) continue;
+    ll_schedule e{k * 1024u, path_, w, h, r, ms};
+    bool replaced = false;
+    for (auto &x : t)
+      if (x.n == e.n) {
+        x = e;
+        replaced = true;
+      }
+    if (!replaced) t.push_back(e);
+  }
+  fclose(f);
+  ll_schedule_table_set((uint32_t)t.size(), t.data());
+}
+
+static bool is_prime_u32(uint32_t n) {
+  if (n < 2) return false;
+  for (uint32_t d = 2; (uint64_t)d * d <= n; d++)
+    if (n % d == 0) return false;
+  return true;
+}
+
+// Largest prime exponent the length-policy bounds give length n (the "max
+// exp" column of fft.txt).
+static uint32_t max_exponent(uint32_t n) {
+  uint32_t lo = n, hi = 40u * n;
+  while (lo + 1 < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    int32_t lb;
+    ll_fft_bounds(mid, &lb, nullptr);
+    if ((uint32_t)lb <= n) lo = mid;
+    else hi = mid;
+  }
+  uint32_t p = lo | 1u;
+  while (p > 3 && !is_prime_u32(p)) p -= 2;
+  return p;
+}
+
+// Average ms per iteration of length n over `passes` passes of 50
+// iterations; *spread receives max - average.
+static double time_length(const Config &cfg, uint32_t n, int passes, double *spread) {
+  ll_options o;
+  memset(&o, 0, sizeof o);
+  o.device = cfg.device < 0 ? 0 : cfg.device;
+  ll_ctx *c = nullptr;
+  const uint32_t p = 17u * n + 1u;
+  std::vector<int32_t> d(n);
+  ll_initial_digits(p, n, 1, d.data());
+  double total = 0, worst = 0;
+  int rc = ll_create(p, n, &o, &c);
+  if (!rc) rc = ll_load_digits(c, d.data(), 1);
+  if (!rc) rc = ll_iterate(c, 64, nullptr, nullptr);
+  for (int k = 0; k < passes && !rc; k++) {
+    rc = ll_iterate(c, 50, nullptr, nullptr);
+    const double t = ll_last_iterate_ms(c) / 50.0;
+    total += t;
+    if (t > worst) worst = t;
+  }
+  ll_destroy(c);
+  if (rc) return -1.0;
+  if (spread) *spread = worst - total / passes;
+  return total / passes;
+}
+
+// cufftbench (CUDALucas.cu:2413-2587): time every supported length in
+// [s, e] K, keep each length that is faster than every longer one (powers of
+// two are always kept), and write them to "<device> fft.txt".
+static int cufftbench(const Config &cfg, int s, int e, int passes) {
+  if (s < 1 || e < s || passes < 1) {
+    fprintf(stderr, "can't parse -cufftbench options\n\n");
+    return 2;
+  }
+  printf("FFT bench, testing supported fft sizes %dK to %dK, doing %d passes.\n", s, e, passes);
+  std::vector<std::pair<int, double>> t;
+  for (int k = s; k <= e; k++) {
+    const uint32_t n = (uint32_t)k * 1024u;
+    if (!ll_fft_supported(n)) continue;
+    double spread = 0;
+    const double ms = time_length(cfg, n, passes, &spread);
+    if (ms < 0) {
+      fprintf(stderr, "fft size = %dK failed\n", k);
+      continue;
+    }
+    printf("fft size = %dK, ave time = %6.4f msec, max-ave = %0.5f\n", k, ms, spread);
+    fflush(stdout);
+    t.emplace_back(k, ms);
+  }
+  std::vector<bool> keep(t.size(), false);
+  double best = 1e30;
+  for (size_t i = t.size(); i-- > 0;) {
+    const bool pow2 = (t[i].first & (t[i].first - 1)) == 0;
+    if (t[i].second < best) {
+      best = t[i].second;
+      keep[i] = true;
+    } else if (pow2) {
+      keep[i] = true;
+    }
+  }
+  const std::string path = device_file(cfg.device, "fft.txt");
+  ll_device_info d;
+  ll_get_device_info(cfg.device < 0 ? 0 : cfg.device, &d);
+  FILE *f = path.empty() ? nullptr : fopen(path.c_str(), "w");
+  FILE *out = f ? f : stdout;
+  if (!f) printf("Cannot open %s.\n", path.c_str());
+  fprintf(out, "Device              %s\n", d.name);
+  fprintf(out, "Architecture        %s\n", d.arch);
+  fprintf(out, "clockRate (MHz)     %d\n", d.clock_mhz);
+  fprintf(out, "memClockRate (MHz)  %d\n", d.mem_clock_mhz);
+  fprintf(out, "\n  fft    max exp  ms/iter\n");
+  for (size_t i = 0; i < t.size(); i++)
+    if (keep[i])
+      fprintf(out, "%5d %10u %8.4f\n", t[i].first, max_exponent((uint32_t)t[i].first * 1024u), t[i].second);
+  if (f) {
+    fclose(f);
+    printf("Optimal fft lengths saved in %s.\n", path.c_str());
+  }
+  return 0;
+}
+
+// threadbench (CUDALucas.cu:2143-2276) for this engine's schedules: every
+// W x H x R (and the LDS-resident kernel) of each length is timed and the
+// fastest appended to "<device> schedules.txt".  mode & 1: only lengths in
+// the FFT-length list; mode & 8: print only the best of each length.
+static int threadbench(const Config &cfg, int s, int e, int passes, int mode) {
+  if (s < 1 || e < s || passes < 1) {
+    fprintf(stderr, "can't parse -threadbench options\n\n");
+    return 2;
+  }
+  printf("Schedule bench, testing every schedule for ffts %dK to %dK, doing %d passes.\n", s, e, passes);
+  const std::string path = device_file(cfg.device, "schedules.txt");
+  FILE *f = path.empty() ? nullptr : fopen(path.c_str(), "a");
+  for (int k = s; k <= e; k++) {
+    const uint32_t n = (uint32_t)k * 1024u;
+    if (mode & 1) {
+      bool listed = false;
+      for (int i = 0; i < ll_fft_length_count(); i++) listed |= (uint32_t)ll_fft_length_at(i) == n;
+      if (!listed) continue;
+    }
+    const int cnt = ll_fft_schedules(n, nullptr, 0);
+    if (cnt <= 0) continue;
+    std::vector<ll_schedule> v(cnt);
+    const int got = ll_tune_length(cfg.device < 0 ? 0 : cfg.device, 0, n, 50u * (uint32_t)passes, v.data(), cnt);
+    if (got <= 0 || v[0].ms < 0) {
+      fprintf(stderr, "fft = %dK: tuning failed\n", k);
+      continue;
+    }
+    if (!(mode & 8))
+      for (const auto &x : v)
+        printf("fft = %dK, %s %u x %u x %u, ave time = %7.4f ms\n", k, x.path ? "sweeps" : "resident", x.width,
+               x.height, x.lines, x.ms);
+    printf("fft = %dK, min time = %7.4f ms, %s %u x %u x %u\n", k, v[0].ms, v[0].path ? "sweeps" : "resident",
+           v[0].width, v[0].height, v[0].lines);
+    if (!(mode & 8)) printf("\n");
+    fflush(stdout);
+    if (f) fprintf(f, "%5d %u %u %u %u %10.5f\n", k, v[0].path, v[0].width, v[0].height, v[0].lines, v[0].ms);
+  }
+  if (f) {
+    fclose(f);
+    printf("Best schedules appended to %s.\n", path.c_str());
+  }
+  return 0;
+}
+
+
 // ------------------------------------------------------------- arguments
 static void usage() {
   fprintf(stderr,
@@ -630,6 +894,12 @@ static void usage() {
           "  -f    set fft length (e.g. 4096K, 4M)\n"
           "  -r    run the residue self-test (0: 20 Mersenne primes, 1: 92 exponents)\n"
           "  -s    save all checkpoint files into folder\n"
+          "  -info print device information\n"
+          "  -cufftbench start end passes: time every supported fft length in [start, end] K and\n"
+          "        save the fastest ones in \"<device name> fft.txt\"\n"
+          "  -threadbench start end passes [mode]: time every kernel schedule of each length and\n"
+          "        append the fastest to \"<device name> schedules.txt\"\n"
+          "  --selector model [--selector-times file]: learned fft length selection\n"
           "  -polite and -k are accepted for compatibility and have no effect.\n");
 }
 
@@ -671,7 +941,21 @@ int main(int argc, char **argv) {
       cfg.selftest = (i + 1 < argc && argv[i + 1][0] != '-') ? atoi(argv[++i]) : 0;
       continue;
     }
-    if (!strcmp(a, "-k") || !strcmp(a, "-info")) continue;
+    if (!strcmp(a, "-info")) { cfg.info = true; continue; }
+    if (!strcmp(a, "-k")) continue;
+    if (!strcmp(a, "-cufftbench")) {
+      if (!need(argc, argv, i, 3)) { fprintf(stderr, "can't parse -cufftbench option\n\n"); return 2; }
+      for (int k = 0; k < 3; k++) cfg.cufftbench[k] = atoi(argv[++i]);
+      continue;
+    }
+    if (!strcmp(a, "-threadbench")) {
+      if (!need(argc, argv, i, 3)) { fprintf(stderr, "can't parse -threadbench option\n\n"); return 2; }
+      for (int k = 0; k < 3; k++) cfg.threadbench[k] = atoi(argv[++i]);
+      if (i + 1 < argc && argv[i + 1][0] != '-') cfg.threadbench[3] = atoi(argv[++i]);
+      continue;
+    }
+    if (!strcmp(a, "--selector") && i + 1 < argc) { cfg.selector_model = argv[++i]; continue; }
+    if (!strcmp(a, "--selector-times") && i + 1 < argc) { cfg.selector_times = argv[++i]; continue; }
     // options without a reference counterpart
     if (!strcmp(a, "--checkpoint-dir") && i + 1 < argc) { cfg.ckpt_dir = argv[++i]; continue; }
     if (!strcmp(a, "--offset") && i + 1 < argc) { cfg.fixed_offset = atoll(argv[++i]); continue; }
@@ -715,6 +999,35 @@ int main(int argc, char **argv) {
   sigprocmask(SIG_UNBLOCK, &quit_set, nullptr);
   signal(SIGINT, on_signal);
   signal(SIGTERM, on_signal);
+  const int dev = cfg.device < 0 ? 0 : cfg.device;
+  if (cfg.info) print_device_info(dev);
+  if (cfg.cufftbench[2] > 0) return cufftbench(cfg, cfg.cufftbench[0], cfg.cufftbench[1], cfg.cufftbench[2]);
+  load_schedule_file(device_file(dev, "schedules.txt"));
+  if (cfg.threadbench[2] > 0)
+    return threadbench(cfg, cfg.threadbench[0], cfg.threadbench[1], cfg.threadbench[2], cfg.threadbench[3]);
+  load_fft_file(device_file(dev, "fft.txt"));
+  if (!cfg.selector_model.empty()) {
+    if (ll_selector_load(dev, cfg.selector_model.c_str(), &g_sel) != LL_OK) {
+      fprintf(stderr, "selector: %s\n", ll_selector_last_error(g_sel));
+      return 2;
+    }
+    if (!cfg.selector_times.empty()) {
+      std::vector<uint32_t> ns;
+      std::vector<float> ms;
+      FILE *f = fopen(cfg.selector_times.c_str(), "r");
+      char buf[256];
+      while (f && fgets(buf, sizeof buf, f)) {
+        unsigned k;
+        float t;
+        if (buf[0] != '#' && sscanf(buf, "%u %f", &k, &t) == 2) {
+          ns.push_back(k * 1024u);
+          ms.push_back(t);
+        }
+      }
+      if (f) fclose(f);
+      ll_selector_set_times(g_sel, (uint32_t)ns.size(), ns.data(), ms.data());
+    }
+  }
   if (cfg.selftest >= 0) return check_residue(cfg, cfg.selftest);
   if (q) {
     // valid_assignment (parse.c:176-203)

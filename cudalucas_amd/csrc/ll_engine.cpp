@@ -30,7 +30,37 @@ struct Schedule {
   const CarryKernel *ck;
 };
 
+// Preferred schedules installed by ll_schedule_table_set (a tuning file).
+std::vector<ll_schedule> &schedule_table() {
+  static std::vector<ll_schedule> t;
+  return t;
+}
+
+bool pick_schedule_auto(uint32_t n, const ll_options *o, Schedule *s);
+
 bool pick_schedule(uint32_t n, const ll_options *o, Schedule *s) {
+  const bool explicit_choice = o && (o->width || o->height || o->lines || o->threads[1] > 0 ||
+                                     (o->flags & LL_OPT_FORCE_SWEEPS));
+  if (!explicit_choice) {
+    for (const ll_schedule &e : schedule_table()) {
+      if (e.n != n) continue;
+      ll_options t;
+      memset(&t, 0, sizeof t);
+      if (o) t = *o;
+      if (e.path == 1) {
+        t.flags |= LL_OPT_FORCE_SWEEPS;
+        t.width = e.width;
+        t.height = e.height;
+        t.lines = e.lines;
+      }
+      if (pick_schedule_auto(n, &t, s)) return true;
+      break;  // stale entry: fall back to the built-in preference
+    }
+  }
+  return pick_schedule_auto(n, o, s);
+}
+
+bool pick_schedule_auto(uint32_t n, const ll_options *o, Schedule *s) {
   memset(s, 0, sizeof(*s));
   if (n < 512 || n % 2) return false;
   const uint32_t M = n / 2;
@@ -858,6 +888,94 @@ int ll_batch_step_errors(const ll_batch *b, uint32_t i, float *errs, uint32_t ca
   if (errs && capacity < e.size()) return LL_ERR_ARG;
   if (errs) memcpy(errs, e.data(), e.size() * sizeof(float));
   return (int)e.size();
+}
+
+int ll_fft_schedules(uint32_t n, ll_schedule *out, uint32_t capacity) {
+  if (n < 512 || n % 2) return 0;
+  const uint32_t M = n / 2;
+  std::vector<ll_schedule> v;
+  if (find_resident(M)) v.push_back({n, 0, M, 1, 1, 0.f});
+  size_t count = 0;
+  const CarryKernel *tab = carry_table(&count);
+  for (size_t i = 0; i < count; i++) {
+    const CarryKernel *ck = &tab[i];
+    if (M % ck->w) continue;
+    const uint32_t H = M / ck->w;
+    if (!find_square(H) || H % ck->r) continue;
+    bool dup = false;
+    for (const auto &e : v) dup |= e.path == 1 && e.width == ck->w && e.lines == ck->r;
+    if (!dup) v.push_back({n, 1, ck->w, H, ck->r, 0.f});
+  }
+  if (out) {
+    if (capacity < v.size()) return LL_ERR_ARG;
+    std::copy(v.begin(), v.end(), out);
+  }
+  return (int)v.size();
+}
+
+int ll_tune_length(int32_t device, uint32_t p, uint32_t n, uint32_t iters, ll_schedule *out, uint32_t capacity) {
+  const int k = ll_fft_schedules(n, nullptr, 0);
+  if (k <= 0) return LL_ERR_UNSUPPORTED;
+  if (!out || capacity < (uint32_t)k || iters == 0) return LL_ERR_ARG;
+  ll_fft_schedules(n, out, capacity);
+  if (p == 0) p = 17u * n + 1u;  // ~17 bits per word: timing does not depend on p
+  std::vector<int32_t> digits(n);
+  ll_initial_digits(p, n, 1, digits.data());
+  for (int i = 0; i < k; i++) {
+    ll_options o;
+    memset(&o, 0, sizeof o);
+    o.device = device;
+    if (out[i].path == 1) {
+      o.flags = LL_OPT_FORCE_SWEEPS;
+      o.width = out[i].width;
+      o.height = out[i].height;
+      o.lines = out[i].lines;
+    }
+    ll_ctx *c = nullptr;
+    int rc = ll_create(p, n, &o, &c);
+    if (!rc) rc = ll_load_digits(c, digits.data(), 1);
+    if (!rc) rc = ll_iterate(c, std::min(iters, 64u), nullptr, nullptr);  // warm-up, graph build
+    if (!rc) rc = ll_iterate(c, iters, nullptr, nullptr);
+    out[i].ms = rc ? -1.f : (float)(ll_last_iterate_ms(c) / iters);
+    ll_destroy(c);
+    if (rc == LL_ERR_HIP) return rc;
+  }
+  std::stable_sort(out, out + k, [](const ll_schedule &a, const ll_schedule &b) {
+    return (a.ms >= 0 ? a.ms : 1e30f) < (b.ms >= 0 ? b.ms : 1e30f);
+  });
+  return k;
+}
+
+int ll_schedule_table_set(uint32_t count, const ll_schedule *table) {
+  if (count && !table) return LL_ERR_ARG;
+  schedule_table().assign(table, table + count);
+  return LL_OK;
+}
+
+int ll_get_device_info(int32_t device, ll_device_info *out) {
+  if (!out) return LL_ERR_ARG;
+  memset(out, 0, sizeof(*out));
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return LL_ERR_ARG;
+  hipDeviceProp_t pr;
+  if (hipGetDeviceProperties(&pr, device) != hipSuccess) return LL_ERR_HIP;
+  snprintf(out->name, sizeof out->name, "%s", pr.name);
+  snprintf(out->arch, sizeof out->arch, "%.63s", pr.gcnArchName);
+  out->compute_units = pr.multiProcessorCount;
+  out->clock_mhz = pr.clockRate / 1000;
+  out->mem_clock_mhz = pr.memoryClockRate / 1000;
+  out->mem_bus_bits = pr.memoryBusWidth;
+  out->global_mem = pr.totalGlobalMem;
+  out->l2_bytes = pr.l2CacheSize;
+  out->lds_per_block = (uint32_t)pr.sharedMemPerBlock;
+  out->max_threads_per_block = pr.maxThreadsPerBlock;
+  out->wavefront = pr.warpSize;
+  out->pci_bus = pr.pciBusID;
+  out->pci_device = pr.pciDeviceID;
+  int v = 0;
+  if (hipRuntimeGetVersion(&v) == hipSuccess) out->runtime_version = v;
+  if (hipDriverGetVersion(&v) == hipSuccess) out->driver_version = v;
+  return LL_OK;
 }
 
 int ll_fft_schedule(uint32_t n, const ll_options *opts, ll_info *info) {

@@ -345,11 +345,36 @@ static const int k_lengths_k[] = {
     41472, 42336, 43200, 43904, 47628, 49000, 50000, 50176, 51200, 52488, 54432, 55296, 56000,
     57344, 60750, 62500, 64000, 64800, 65536};
 
-int ll_fft_length_count(void) { return (int)(sizeof(k_lengths_k) / sizeof(k_lengths_k[0])); }
+// The active list: the defaults, or one installed by ll_fft_length_list_set
+// (the analog of init_ffts merging "<device> fft.txt", CUDALucas.cu:947-1025).
+static std::vector<int> &length_list() {
+  static std::vector<int> v = [] {
+    std::vector<int> d;
+    for (int k : k_lengths_k) d.push_back(k * 1024);
+    return d;
+  }();
+  return v;
+}
+
+int ll_fft_length_list_set(uint32_t count, const uint32_t *lengths) {
+  std::vector<int> &v = length_list();
+  if (count == 0) {
+    v.clear();
+    for (int k : k_lengths_k) v.push_back(k * 1024);
+    return LL_OK;
+  }
+  if (!lengths) return LL_ERR_ARG;
+  for (uint32_t i = 0; i < count; i++)
+    if (lengths[i] == 0 || lengths[i] > 0x7fffffffu || (i && lengths[i] <= lengths[i - 1])) return LL_ERR_ARG;
+  v.assign(lengths, lengths + count);
+  return LL_OK;
+}
+
+int ll_fft_length_count(void) { return (int)length_list().size(); }
 
 int ll_fft_length_at(int index) {
   if (index < 0 || index >= ll_fft_length_count()) return -1;
-  return k_lengths_k[index] * 1024;
+  return length_list()[index];
 }
 
 void ll_fft_bounds(uint32_t p, int32_t *lb, int32_t *ub) {

@@ -24,6 +24,7 @@ namespace {
 constexpr uint32_t kMaxLengthK = 16384;  // candidates: multiples of 1K up to 16M words
 constexpr double kDomainLow = 0.62;      // candidates from 0.62 n_ref ...
 constexpr double kDomainHigh = 1.25;     // ... (training also covers up to 1.25 n_ref)
+constexpr double kMaxBitsPerWord = 28.0; // and never past 28 bits per word
 
 const std::vector<uint32_t> &supported_lengths() {
   static std::vector<uint32_t> v = [] {
@@ -86,7 +87,7 @@ int ll_selector_candidates(uint32_t p, uint32_t *ns, uint32_t capacity) {
   int idx = -1;
   const int nref = ll_choose_fft_length(p, &idx);
   if (nref <= 0) return LL_ERR_UNSUPPORTED;
-  const double lo = kDomainLow * nref, hi = kDomainHigh * nref;
+  const double lo = std::max(kDomainLow * nref, p / kMaxBitsPerWord), hi = kDomainHigh * nref;
   uint32_t k = 0;
   for (uint32_t n : supported_lengths()) {
     if (n < lo || n > hi) continue;
@@ -211,7 +212,7 @@ int ll_selector_choose(ll_selector *s, uint32_t p, uint32_t above, float *pred_e
   if (nref <= 0) return sfail(s, LL_ERR_UNSUPPORTED, "no length for p=%u", p);
   std::vector<uint32_t> cand;
   for (uint32_t n : supported_lengths()) {
-    if (n < kDomainLow * nref || n <= above) continue;
+    if (n < kDomainLow * nref || n < p / kMaxBitsPerWord || n <= above) continue;
     if (!above && n > (uint32_t)nref) break;  // first choice: never longer than the reference's
     cand.push_back(n);
   }

@@ -111,3 +111,5 @@ C5_TRAIN_SEED = 5005
 C5_EVAL_SEED = 5006
 C5_EVAL_COUNT = 512
 C5_EVAL_RANGE = (100_000, 20_000_000)
+C5_TRAIN_COUNT = 20_000
+C5_TRAIN_RANGE = (100_000, 120_000_000)

@@ -180,8 +180,8 @@ typedef struct ll_selector ll_selector;
 /* Writes LL_SELECTOR_FEATURES raw features; returns their count or < 0. */
 int ll_selector_features(uint32_t p, uint32_t n, float *features);
 /* Lengths the selector considers for p: supported multiples of 1K words in
- * [0.62, 1.25] x the reference policy's length. Returns the count (ns may
- * be NULL to query it). */
+ * [0.62, 1.25] x the reference policy's length, at most 28 bits per word.
+ * Returns the count (ns may be NULL to query it). */
 int ll_selector_candidates(uint32_t p, uint32_t *ns, uint32_t capacity);
 /* Loads a text model ("cudalucas_amd roundoff-mlp 1") onto `device`. */
 int ll_selector_load(int32_t device, const char *path, ll_selector **out);
@@ -262,6 +262,46 @@ int ll_fft_supported(uint32_t n);
  * height = lines = 1). LL_ERR_UNSUPPORTED if there is none. The analog of
  * the reference's thread/plan tables (CUDALucas.cu:812-945). */
 int ll_fft_schedule(uint32_t n, const ll_options *opts, ll_info *info);
+
+/* ---- schedule tuner (the analog of -threadbench, CUDALucas.cu:2143-2276,
+ * whose "<device> threads.txt" init_threads reads back, :812-945) -------- */
+typedef struct ll_schedule {
+  uint32_t n;                    /* FFT length (words) */
+  uint32_t path;                 /* 0 = LDS-resident, 1 = two sweeps */
+  uint32_t width, height, lines; /* W, H, R (resident: n/2, 1, 1) */
+  float ms;                      /* measured ms per iteration (< 0: failed) */
+} ll_schedule;
+/* Every schedule this build has for length n; returns the count (out may
+ * be NULL to query it). */
+int ll_fft_schedules(uint32_t n, ll_schedule *out, uint32_t capacity);
+/* Times every schedule of length n over `iters` hipGraph-replayed
+ * iterations (after a warm-up) at exponent p (0: about 17 bits per word);
+ * out receives them fastest first. Returns the count. */
+int ll_tune_length(int32_t device, uint32_t p, uint32_t n, uint32_t iters, ll_schedule *out,
+                   uint32_t capacity);
+/* Installs preferred schedules (e.g. read back from a tuning file): ll_create
+ * and ll_batch_create use the entry for n unless the options name a schedule
+ * (width/height/lines, -threads, LL_OPT_FORCE_SWEEPS). count = 0 clears. */
+int ll_schedule_table_set(uint32_t count, const ll_schedule *table);
+
+/* Installs the FFT-length list that ll_fft_length_at / ll_choose_fft_length
+ * use (ascending, words), as init_ffts merges "<device> fft.txt" into the
+ * defaults (CUDALucas.cu:947-1025). count = 0 restores the 162 defaults. */
+int ll_fft_length_list_set(uint32_t count, const uint32_t *lengths);
+
+/* Device properties for -info and the tuning files (init_device,
+ * CUDALucas.cu:737-810). */
+typedef struct ll_device_info {
+  char name[256];
+  char arch[64];
+  int32_t compute_units, clock_mhz, mem_clock_mhz, mem_bus_bits;
+  uint64_t global_mem;
+  int32_t l2_bytes;
+  uint32_t lds_per_block;
+  int32_t max_threads_per_block, wavefront, pci_bus, pci_device;
+  int32_t runtime_version, driver_version;
+} ll_device_info;
+int ll_get_device_info(int32_t device, ll_device_info *out);
 /* choose_fft_length (CUDALucas.cu:1027-1068): *index < 0 selects from the
  * list by the lb/ub power laws; *index >= count is a manual length.  Lengths
  * without kernels in this build are skipped upward. Returns n or < 0. */
