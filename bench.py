@@ -77,19 +77,27 @@ def batched_leg(steps=4, step_iters=512, plain_iters=200):
     import numpy as np
 
     import cudalucas_amd as L
+    from cudalucas_amd import tuner as T
     from cudalucas_amd import workloads as W
 
     ps = W.log_uniform_primes(W.C4_SEED, W.C4_COUNT, *W.C4_RANGE)
     lens = [L.choose_fft_length(p)[0] for p in ps]
     tests = [(p, n, p % 1000, p - 2) for p, n in zip(ps, lens)]
-    with L.LLBatch(tests) as b:
-        b.step(64)
-        total_ms, worst = 0.0, 0.0
-        for _ in range(steps):
-            e = b.step(step_iters)
-            total_ms += b.last_step_ms()
-            worst = max(worst, float(e.max()))
-        groups = b.groups()
+    tuned = os.path.exists(T.BATCH_TABLE)
+    if tuned:  # schedules picked by the batched autotuner
+        T.install(T.load(T.BATCH_TABLE))
+    try:
+        with L.LLBatch(tests) as b:
+            b.step(64)
+            total_ms, worst = 0.0, 0.0
+            for _ in range(steps):
+                e = b.step(step_iters)
+                total_ms += b.last_step_ms()
+                worst = max(worst, float(e.max()))
+            groups = b.groups()
+    finally:
+        if tuned:
+            T.install([])
     t_step = total_ms * 1e-3 / (steps * step_iters)
     mean_iters = float(np.mean([p - 2 for p in ps]))
     batched = 3600.0 * len(ps) / (t_step * mean_iters)
@@ -110,6 +118,7 @@ def batched_leg(steps=4, step_iters=512, plain_iters=200):
         "batch_iteration_ms": round(t_step * 1e3, 4),
         "sample": "%d timed batch iterations after 64 warm-up; plain: %d iterations per exponent, all 256"
         % (steps * step_iters, plain_iters),
+        "schedules": "batched autotuner table" if tuned else "built-in preference",
         "max_roundoff": round(worst, 4),
     }
 

@@ -134,6 +134,11 @@ def lib():
     L.ll_fft_bounds.restype = None
     L.ll_fft_from_str.argtypes = [ctypes.c_char_p]
     L.ll_fft_schedule.argtypes = [u32, ctypes.POINTER(_Options), ctypes.POINTER(_Info)]
+    L.ll_fft_length_list_set.argtypes = [u32, P]
+    L.ll_fft_schedules.argtypes = [u32, P, u32]
+    L.ll_tune_length.argtypes = [i32, u32, u32, u32, P, u32]
+    L.ll_schedule_table_set.argtypes = [u32, P]
+    L.ll_get_device_info.argtypes = [i32, P]
     L.ll_batch_step_errors.argtypes = [P, u32, P, u32]
     L.ll_mlp_create.argtypes = [i32, u32, P, ctypes.POINTER(P)]
     L.ll_mlp_destroy.argtypes = [P]

@@ -317,4 +317,92 @@ __device__ __forceinline__ void fft_lds(cplx *buf, const TW &tw) {
   }
 }
 
+// ------------------------------------------------ per-thread twiddle cache
+// In pass PASS of an NB x L Stockham transform run by T threads, thread t
+// always handles butterflies u = t + s*T, so its twiddles w^(i*(j mod NS))
+// are fixed: they are computed once per kernel and shared by the forward
+// and the inverse transform (the inverse uses their conjugates), which
+// takes the table lookups out of the passes.
+template <int L, int NB, int T, int PASS = 0, int NS = 1, bool END = (NS >= L)>
+struct PassTwiddles;
+
+template <int L, int NB, int T, int PASS, int NS>
+struct PassTwiddles<L, NB, T, PASS, NS, true> {
+  template <typename TW>
+  __device__ __forceinline__ void init(const TW &) {}
+};
+
+template <int L, int NB, int T, int PASS, int NS>
+struct PassTwiddles<L, NB, T, PASS, NS, false> {
+  static constexpr int R = plan_radix(L, PASS);
+  static constexpr int Q = L / R;
+  static constexpr int NBF = NB * Q;
+  static constexpr int PER = (NBF + T - 1) / T;
+  static constexpr int NW = NS > 1 ? R - 1 : 1;
+  cplx w[PER][NW];
+  PassTwiddles<L, NB, T, PASS + 1, NS * R> next;
+
+  template <typename TW>
+  __device__ __forceinline__ void init(const TW &tw) {
+    if constexpr (NS > 1) {
+#pragma unroll
+      for (int s = 0; s < PER; s++) {
+        const int u = (int)threadIdx.x + s * T;
+        const int k = (u % Q) % NS;
+#pragma unroll
+        for (int i = 1; i < R; i++) w[s][i - 1] = tw((i * k) * (L / (NS * R)));
+      }
+    }
+    next.init(tw);
+  }
+};
+
+// stockham_pass with the thread's twiddles taken from a PassTwiddles level.
+template <int L, int R, int NS, int NB, int T, bool INV, typename PT>
+__device__ __forceinline__ void stockham_pass_cached(cplx *buf, const PT &pt) {
+  constexpr int Q = L / R;
+  constexpr int NBF = NB * Q;
+  constexpr int PER = (NBF + T - 1) / T;
+  constexpr int LP = lpitch(L);
+  cplx v[PER][R];
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int s = 0; s < PER; s++) {
+    const int u = tid + s * T;
+    if ((NBF % T == 0) || u < NBF) {
+      const int b = u / Q, j = u % Q;
+#pragma unroll
+      for (int i = 0; i < R; i++) {
+        cplx a = buf[b * LP + lpad(j + i * Q)];
+        if (NS > 1 && i > 0) a = INV ? cmulc(a, pt.w[s][i - 1]) : cmul(a, pt.w[s][i - 1]);
+        v[s][i] = a;
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < PER; s++) {
+    const int u = tid + s * T;
+    if ((NBF % T == 0) || u < NBF) {
+      const int b = u / Q, j = u % Q;
+      const int k = j % NS;
+      dft<R, INV>(v[s]);
+      const int base = (j / NS) * NS * R + k;
+#pragma unroll
+      for (int i = 0; i < R; i++) buf[b * LP + lpad(base + i * NS)] = v[s][i];
+    }
+  }
+  __syncthreads();
+}
+
+// fft_lds with cached twiddles (pt: the PassTwiddles of the same L, NB, T).
+template <int L, int NB, int T, bool INV, int PASS = 0, int NS = 1, typename PT>
+__device__ __forceinline__ void fft_lds_cached(cplx *buf, const PT &pt) {
+  if constexpr (NS < L) {
+    constexpr int R = plan_radix(L, PASS);
+    stockham_pass_cached<L, R, NS, NB, T, INV>(buf, pt);
+    fft_lds_cached<L, NB, T, INV, PASS + 1, NS * R>(buf, pt.next);
+  }
+}
+
 }  // namespace llamd

@@ -174,7 +174,7 @@ __global__ void fill_u32(uint32_t *p, uint32_t v, size_t n) {
 #define LL_RESIDENT(M, T) {M, (const void *)resident_kernel<M, T>, T}
 
 static const SquareKernel k_square[] = {
-    LL_SQUARE_T(128, 64), LL_SQUARE(256), LL_SQUARE(512), LL_SQUARE(1024), LL_SQUARE(2048), LL_SQUARE(4096),
+    LL_SQUARE_T(128, 64), LL_SQUARE(256), LL_SQUARE(512), LL_SQUARE(1024), LL_SQUARE(2048), LL_SQUARE_T(4096, 512),
 };
 
 // Carry-sweep instantiations live in ll_carry_*.hip (compiled in parallel).

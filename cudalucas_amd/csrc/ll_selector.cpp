@@ -25,6 +25,7 @@ constexpr uint32_t kMaxLengthK = 16384;  // candidates: multiples of 1K up to 16
 constexpr double kDomainLow = 0.62;      // candidates from 0.62 n_ref ...
 constexpr double kDomainHigh = 1.25;     // ... (training also covers up to 1.25 n_ref)
 constexpr double kMaxBitsPerWord = 28.0; // and never past 28 bits per word
+constexpr double kLongerLimit = 2.0;     // timed choices may go up to 2 n_ref
 
 const std::vector<uint32_t> &supported_lengths() {
   static std::vector<uint32_t> v = [] {
@@ -55,6 +56,7 @@ struct ll_selector {
   float window = 128.f;  // iterations the training target's maximum covers
   float margin = 0.f;    // log2 units added to every prediction
   float limit = 0.35f;   // acceptable full-test maximum roundoff
+  float growth_scale = 1.f;  // measured / Gaussian-model growth of the maximum
   std::vector<std::pair<uint32_t, float>> times;  // (n, ms/iter), sorted by n
   char msg[256] = {0};
 };
@@ -138,6 +140,9 @@ int ll_selector_load(int32_t device, const char *path, ll_selector **out) {
       if (fscanf(f, "%f %f", &s->target_mean, &s->target_scale) != 2) rc = sfail(s, LL_ERR_ARG, "bad target");
     } else if (!strcmp(key, "window")) {
       if (fscanf(f, "%f", &s->window) != 1 || s->window < 1) rc = sfail(s, LL_ERR_ARG, "bad window");
+    } else if (!strcmp(key, "growth_scale")) {
+      if (fscanf(f, "%f", &s->growth_scale) != 1 || !(s->growth_scale >= 1.f && s->growth_scale < 4.f))
+        rc = sfail(s, LL_ERR_ARG, "bad growth_scale");
     } else if (!strcmp(key, "margin")) {
       if (fscanf(f, "%f", &s->margin) != 1) rc = sfail(s, LL_ERR_ARG, "bad margin");
     } else if (!strcmp(key, "limit")) {
@@ -200,7 +205,7 @@ int ll_selector_predict(ll_selector *s, uint32_t p, uint32_t count, const uint32
     // extrapolate from the training window to the p - 2 iterations of a test
     const double n = ns[i];
     const double growth = 0.5 * log2(log(n * (double)(p - 2)) / log(n * (double)s->window));
-    log2_err[i] = y[i] * s->target_scale + s->target_mean + (float)std::max(0.0, growth) + s->margin;
+    log2_err[i] = y[i] * s->target_scale + s->target_mean + s->growth_scale * (float)std::max(0.0, growth) + s->margin;
   }
   return LL_OK;
 }
@@ -210,17 +215,28 @@ int ll_selector_choose(ll_selector *s, uint32_t p, uint32_t above, float *pred_e
   int idx = -1;
   const int nref = ll_choose_fft_length(p, &idx);
   if (nref <= 0) return sfail(s, LL_ERR_UNSUPPORTED, "no length for p=%u", p);
-  std::vector<uint32_t> cand;
+  // With a timing table every length of the trained domain competes on
+  // measured speed, and so do the longer lengths up to twice the reference's
+  // (fewer bits per word than anything the reference would use: safe without
+  // a prediction); without one, the first choice is the shortest safe length
+  // no longer than the reference's.
+  const bool timed = !s->times.empty();
+  std::vector<uint32_t> cand, longer;
   for (uint32_t n : supported_lengths()) {
     if (n < kDomainLow * nref || n < p / kMaxBitsPerWord || n <= above) continue;
-    if (!above && n > (uint32_t)nref) break;  // first choice: never longer than the reference's
-    cand.push_back(n);
+    if (!above && n > (timed ? kLongerLimit * nref : (double)nref)) break;
+    if (n > kDomainHigh * nref && !above) longer.push_back(n);
+    else cand.push_back(n);
   }
   if (cand.empty()) return sfail(s, LL_ERR_UNSUPPORTED, "no supported length above %u", above);
   std::vector<float> pred(cand.size());
   int rc = ll_selector_predict(s, p, (uint32_t)cand.size(), cand.data(), pred.data());
   if (rc) return rc;
   const float lim = log2f(s->limit);
+  for (uint32_t n : longer) {  // accepted without a prediction
+    cand.push_back(n);
+    pred.push_back(-INFINITY);
+  }
   int best = -1;
   double best_t = 1e30;
   for (size_t i = 0; i < cand.size(); i++) {
@@ -245,7 +261,7 @@ int ll_selector_choose(ll_selector *s, uint32_t p, uint32_t above, float *pred_e
     }
     return (int)n;
   }
-  if (pred_err) *pred_err = exp2f(pred[best]);
+  if (pred_err) *pred_err = pred[best] == -INFINITY ? 0.f : exp2f(pred[best]);
   return (int)cand[best];
 }
 

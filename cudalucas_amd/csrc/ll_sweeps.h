@@ -12,12 +12,13 @@
 //    m = a*H + b holds words (x[2m], x[2m+1]); lines b are tiled by R, so the
 //    element lives at ((b / R) W + a) R + (b mod R): a carry workgroup's R
 //    lines are one contiguous block, a column is read in runs of R.
-//      square_sweep : per column pair (k1, W-k1): forward H-point DFT,
-//                     real-pairing squaring, inverse H-point DFT.
-//      carry_sweep  : per block of R lines b: inverse twiddle, inverse W-point
-//                     DFT, unweight/round/-2/balanced carry, carry look-back
+//      square_sweep : per column pair (k1, W-k1): four-step twiddle,
+//                     forward H-point DFT, real-pairing squaring, inverse
+//                     H-point DFT, inverse twiddle.
+//      carry_sweep  : per block of R lines b: inverse W-point DFT,
+//                     unweight/round/-2/balanced carry, carry look-back
 //                     from the previous block, reweight, forward W-point
-//                     DFT, forward twiddle.
+//                     DFT.
 //    Each sweep reads and writes the signal once: 32 N bytes per iteration.
 //
 // Carries between line blocks travel through an epoch-tagged shuttle
@@ -25,10 +26,9 @@
 // half) written and polled with agent-scope atomics, so no fence or flag is
 // needed.  Blocks take tickets from an atomic counter, so a block only waits
 // on blocks that were admitted before it.  The cyclic wrap (the last block's
-// carries feed block 0) is handled by admitting H/R + 1 workgroups per
-// launch: ticket 0 computes block 0 only to publish its carries; the last
-// ticket recomputes block 0 from the untouched input, applies the wrapped
-// carries and writes it.
+// carries feed block 0, one column to the left) is the one wait on a later
+// ticket; it cannot deadlock because every block publishes its own carries
+// before it waits, so the last block never waits on block 0.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -164,14 +164,28 @@ __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict_
   const cplx *col0 = A.x + ((size_t)c0 << rs);
   const cplx *col1 = A.x + ((size_t)c1 << rs);
   load_quarter<H, T>(qc, A.qcH);
-  for (int i = tid; i < H; i += T) {
-    const size_t off = ((size_t)(i >> rs) * W << rs) | (i & rmask);
-    lds[lpad(i)] = col0[off];
-    lds[LP + lpad(i)] = col1[off];
+  // The four-step twiddle w_M^(c b) between the line and the column DFTs
+  // is applied here on the way in and undone on the way out.  Element
+  // b = tid + s T of column c takes w^(c tid) * w^(c s T): two table roots
+  // per thread and column instead of one per element.
+  static_assert(H % T == 0, "square sweep: H must be a multiple of T");
+  {
+    const cplx t0 = twiddle_m(A.twM, c0 * (uint32_t)tid), t1 = twiddle_m(A.twM, c1 * (uint32_t)tid);
+#pragma unroll 2
+    for (int st = 0; st < H / T; st++) {
+      const int i = tid + st * T;
+      const size_t off = ((size_t)(i >> rs) * W << rs) | (i & rmask);
+      const cplx w0 = cmul(t0, twiddle_m(A.twM, c0 * (uint32_t)(st * T)));
+      const cplx w1 = cmul(t1, twiddle_m(A.twM, c1 * (uint32_t)(st * T)));
+      lds[lpad(i)] = cmul(col0[off], w0);
+      lds[LP + lpad(i)] = cmul(col1[off], w1);
+    }
   }
   __syncthreads();
   const QuarterRoots<H> tw{qc};
-  fft_lds<H, 2, T, false>(lds, tw);
+  PassTwiddles<H, 2, T> ptw;
+  ptw.init(tw);
+  fft_lds_cached<H, 2, T, false>(lds, ptw);
   if (g != 0) {
     for (int k2 = tid; k2 < H; k2 += T) {
       const int k2b = H - 1 - k2;
@@ -198,13 +212,20 @@ __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict_
     }
   }
   __syncthreads();
-  fft_lds<H, 2, T, true>(lds, tw);
+  fft_lds_cached<H, 2, T, true>(lds, ptw);
   cplx *out0 = A.x + ((size_t)c0 << rs);
   cplx *out1 = A.x + ((size_t)c1 << rs);
-  for (int i = tid; i < H; i += T) {
-    const size_t off = ((size_t)(i >> rs) * W << rs) | (i & rmask);
-    out0[off] = lds[lpad(i)];
-    out1[off] = lds[LP + lpad(i)];
+  {
+    const cplx t0 = twiddle_m(A.twM, c0 * (uint32_t)tid), t1 = twiddle_m(A.twM, c1 * (uint32_t)tid);
+#pragma unroll 2
+    for (int st = 0; st < H / T; st++) {
+      const int i = tid + st * T;
+      const size_t off = ((size_t)(i >> rs) * W << rs) | (i & rmask);
+      const cplx w0 = cmul(t0, twiddle_m(A.twM, c0 * (uint32_t)(st * T)));
+      const cplx w1 = cmul(t1, twiddle_m(A.twM, c1 * (uint32_t)(st * T)));
+      out0[off] = cmulc(lds[lpad(i)], w0);
+      out1[off] = cmulc(lds[LP + lpad(i)], w1);
+    }
   }
 }
 
@@ -251,12 +272,10 @@ __global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArg
   const Minus2 m2 = s_m2;
   const QuarterRoots<W> tw{qc};
 
-  // load R lines and undo the column twiddle
+  // load R lines (the square sweep has already undone the column twiddle)
   for (int e = tid; e < R * W; e += T) {
     const int r = e % R, k1 = e / R;
-    cplx v = A.x[(size_t)b0 * W + e];
-    v = cmulc(v, twiddle_m(A.twM, (b0 + r) * (uint32_t)k1));
-    lds[r * LP + lpad(k1)] = v;
+    lds[r * LP + lpad(k1)] = A.x[(size_t)b0 * W + e];
   }
   __syncthreads();
   fft_lds<W, R, T, true>(lds, tw);
@@ -345,8 +364,7 @@ __global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArg
   fft_lds<W, R, T, false>(lds, tw);
   for (int e = tid; e < R * W; e += T) {
     const int r = e % R, k1 = e / R;
-    const cplx v = lds[r * LP + lpad(k1)];
-    A.x[(size_t)b0 * W + e] = cmul(v, twiddle_m(A.twM, (b0 + r) * (uint32_t)k1));
+    A.x[(size_t)b0 * W + e] = lds[r * LP + lpad(k1)];
   }
   const float m = block_max<T>((float)emax, red);
   if (tid == 0) {
@@ -388,8 +406,7 @@ __global__ __launch_bounds__(T) void load_lines(const PlanArgs *__restrict__ arg
   fft_lds<W, R, T, false>(lds, tw);
   for (int e = tid; e < R * W; e += T) {
     const int r = e % R, k1 = e / R;
-    const cplx v = lds[r * LP + lpad(k1)];
-    A.x[(size_t)b0 * W + e] = cmul(v, twiddle_m(A.twM, (b0 + r) * (uint32_t)k1));
+    A.x[(size_t)b0 * W + e] = lds[r * LP + lpad(k1)];
   }
 }
 
@@ -408,8 +425,7 @@ __global__ __launch_bounds__(T) void unload_lines(const PlanArgs *__restrict__ a
   const uint32_t b0 = blockIdx.x * R;
   for (int e = tid; e < R * W; e += T) {
     const int r = e % R, k1 = e / R;
-    cplx v = A.x[(size_t)b0 * W + e];
-    lds[r * LP + lpad(k1)] = cmulc(v, twiddle_m(A.twM, (b0 + r) * (uint32_t)k1));
+    lds[r * LP + lpad(k1)] = A.x[(size_t)b0 * W + e];
   }
   __syncthreads();
   fft_lds<W, R, T, true>(lds, tw);

@@ -19,12 +19,17 @@ Pipeline (every GPU step goes through libcudalucas_amd.so):
    then 8 windows of 16 iterations, each window's maximum roundoff recorded.
 3. ``fit``: target log2(max over the 128 iterations), clipped to
    [2^-24, 0.5]; standardised features; MLP 7-64-64-64-1 trained with Adam
-   on the GPU; margin = 99.5 % quantile of held-out residuals near the limit.
+   on the GPU; margin = 99.5 % quantile of held-out residuals near the limit;
+   growth_scale = how much faster the maximum grew from 16 to 128 iterations
+   than the Gaussian model says (never below 1).
 4. ``save_model`` writes the text model that ``ll_selector_load`` reads.
 5. ``evaluate``: selector choice -> careful round-off test in the batched
    engine -> fallback to the next length on failure; residues at the end of
    the test compared with the same iterations at the reference policy's
-   length; projected full-test time from measured ms/iteration per length.
+   length; projected one-at-a-time time from measured ms/iteration per
+   length (against the reference policy on its default list and on the list
+   its own cufftbench rule would keep), and measured batched throughput at
+   the reference's and at the selected lengths.
 
     python -m cudalucas_amd.selector traces --out traces.npz
     python -m cudalucas_amd.selector train --traces traces.npz --model model.txt
@@ -48,6 +53,7 @@ from . import workloads as W
 _HERE = os.path.dirname(os.path.abspath(__file__))
 MODEL_PATH = os.path.join(_HERE, "data", "roundoff_mlp.txt")
 TIMES_PATH = os.path.join(_HERE, "data", "mi355x_fft_times.txt")
+BATCH_TIMES_PATH = os.path.join(_HERE, "data", "mi355x_batched_times.txt")
 
 FEATURES = 7
 SIZES = [FEATURES, 64, 64, 64, 1]
@@ -177,6 +183,7 @@ def fit(ps, ns, wmax, seed=7, epochs=(200, 100, 100), lrs=(2e-3, 5e-4, 1e-4), ba
     ok = (wm.max(axis=1) < 0.4) & (wm[:, 0] > 1e-6)
     obs = np.log2(wm[ok].max(axis=1) / wm[ok, 0])
     pre = growth(ns[ok], WINDOWS * WINDOW_ITERS, WINDOW_ITERS)
+    gscale = float(max(1.0, obs.mean() / pre.mean())) if len(obs) else 1.0
     stats = {
         "samples": int(len(y)),
         "train": int(len(tr)),
@@ -189,6 +196,7 @@ def fit(ps, ns, wmax, seed=7, epochs=(200, 100, 100), lrs=(2e-3, 5e-4, 1e-4), ba
         "saturated_fraction": float(np.mean(y >= -1.0 - 1e-9)),
         "growth_16_to_128_observed_mean": float(obs.mean()) if len(obs) else None,
         "growth_16_to_128_predicted_mean": float(pre.mean()) if len(obs) else None,
+        "growth_scale": gscale,
         "loss_curve": [losses[0], losses[len(losses) // 2], losses[-1]],
     }
     model = {
@@ -198,6 +206,7 @@ def fit(ps, ns, wmax, seed=7, epochs=(200, 100, 100), lrs=(2e-3, 5e-4, 1e-4), ba
         "target": [tmean, tscale],
         "window": WINDOWS * WINDOW_ITERS,
         "margin": margin,
+        "growth_scale": gscale,
         "limit": LIMIT,
         "params": [float(v) for v in params],
     }
@@ -215,6 +224,7 @@ def save_model(model, path, comment=""):
         f.write("target %.9g %.9g\n" % tuple(model["target"]))
         f.write("window %d\n" % model["window"])
         f.write("margin %.6f\n" % model["margin"])
+        f.write("growth_scale %.6f\n" % model.get("growth_scale", 1.0))
         f.write("limit %.4f\n" % model["limit"])
         ps = model["params"]
         f.write("params %d\n" % len(ps))
@@ -248,7 +258,7 @@ def read_model(path):
         elif k == "target":
             model[k] = [float(toks[i]), float(toks[i + 1])]
             i += 2
-        elif k in ("window", "margin", "limit"):
+        elif k in ("window", "margin", "limit", "growth_scale"):
             model[k] = float(toks[i])
             i += 1
         elif k == "params":
@@ -389,11 +399,40 @@ def _run_judged(tests, iters, words_budget=1 << 27, max_tests=256):
     return out
 
 
-def evaluate(selector, primes, times, judge_iters=JUDGE_ITERS, seed=W.C5_EVAL_SEED, log=None):
-    """Selector vs the reference policy on `primes` (see module doc)."""
-    rng = W.SplitMix64(seed ^ 0x0FF5E7)
-    offs = [rng.next() % p for p in primes]
-    nref = [choose_fft_length(p)[0] for p in primes]
+def tuned_reference_lengths(times):
+    """The length list the reference's own tuning would leave on this GPU:
+    cufftbench's rule (CUDALucas.cu:2505-2522) keeps each length faster than
+    every longer one, and every power of two (in K), and init_ffts uses the
+    kept lengths (CUDALucas.cu:947-1025)."""
+    keep, best = [], float("inf")
+    for n in sorted(times, reverse=True):
+        k = n // 1024
+        if times[n] < best:
+            best = times[n]
+            keep.append(n)
+        elif k & (k - 1) == 0:
+            keep.append(n)
+    return sorted(keep)
+
+
+def reference_choices(primes, lengths=None):
+    """choose_fft_length over the default list, or over `lengths`."""
+    L = lib()
+    if lengths:
+        arr = np.asarray(lengths, dtype=np.uint32)
+        if L.ll_fft_length_list_set(len(arr), _ptr(arr)):
+            raise LLError(-1, "ll_fft_length_list_set")
+    try:
+        return [choose_fft_length(p)[0] for p in primes]
+    finally:
+        if lengths:
+            L.ll_fft_length_list_set(0, None)
+
+
+def _select_and_judge(selector, primes, offs, times, judge_iters, log=None, tag=""):
+    """First choices, the round-off judge, fallbacks until every exponent
+    has an accepted length.  Returns (first, accepted, fallbacks, results,
+    wasted ms) with results[i] = (max err, mean err, res64, digest)."""
     cur = [selector.choose(p)[0] for p in primes]
     first = list(cur)
     accepted = [None] * len(primes)
@@ -417,35 +456,112 @@ def evaluate(selector, primes, times, judge_iters=JUDGE_ITERS, seed=W.C5_EVAL_SE
                 nxt.append(i)
         pending = nxt
         if log:
-            log("evaluation round %d: %d tests failed the round-off test" % (rounds, len(nxt)))
+            log("%s round %d: %d exponents failed the round-off test" % (tag, rounds, len(nxt)))
     if pending:
         raise RuntimeError("selector did not settle for %d exponents" % len(pending))
-    ref = _run_judged([(p, n, o) for p, n, o in zip(primes, nref, offs)], judge_iters)
+    return first, accepted, fallbacks, result, wasted_ms
+
+
+def _batch_iteration_ms(tests, iters, words_budget=1 << 27, max_tests=256):
+    """ms of one batched iteration over all (p, n, offset) tests (chunks run
+    one after another)."""
+    order = sorted(range(len(tests)), key=lambda i: tests[i][1])
+    total = 0.0
+    for chunk in _chunks([(i, tests[i][1]) for i in order], words_budget, max_tests):
+        batch = [(tests[i][0], tests[i][1], tests[i][2], 64 + iters) for i, _ in chunk]
+        with LLBatch(batch) as b:
+            b.step(64)
+            b.step(iters)
+            total += b.last_step_ms() / iters
+    return total
+
+
+def evaluate(model_path, primes, times, batch_times=None, schedules=None, judge_iters=JUDGE_ITERS,
+             batch_iters=512, seed=W.C5_EVAL_SEED, log=None):
+    """The learned selector against the reference policy on `primes`.
+
+    Two uses of the same model, each through the round-off judge with
+    fallback, each with residues compared with the reference-length run:
+
+    * "timed": the fastest length predicted safe by the plain engine's
+      measured ms/iteration (one exponent at a time), projected full-test
+      hours against the reference policy on its default list and on the list
+      its own cufftbench rule keeps;
+    * "batched": the cheapest length predicted safe by the batched engine's
+      per-test cost table (`batch_times`; without one, the shortest safe
+      length), with the batched iteration time measured at the reference's
+      and at the selected lengths (`schedules`: the tuned batched schedule
+      table, installed for both)."""
+    from . import tuner
+
+    rng = W.SplitMix64(seed ^ 0x0FF5E7)
+    offs = [rng.next() % p for p in primes]
+    nref = reference_choices(primes)
+    ntuned = reference_choices(primes, tuned_reference_lengths(times))
+    if schedules:
+        tuner.install(schedules)
+    try:
+        with Selector(model_path, times=times) as st:
+            t_first, t_acc, t_fb, t_res, t_waste = _select_and_judge(st, primes, offs, times, judge_iters, log, "timed")
+        with Selector(model_path, times=batch_times) as sb:
+            b_first, b_acc, b_fb, b_res, _ = _select_and_judge(sb, primes, offs, times, judge_iters, log, "batched")
+        ref = _run_judged([(p, n, o) for p, n, o in zip(primes, nref, offs)], judge_iters)
+        b_ref = _batch_iteration_ms([(p, n, o) for p, n, o in zip(primes, nref, offs)], batch_iters)
+        b_sel = _batch_iteration_ms([(p, n, o) for p, n, o in zip(primes, b_acc, offs)], batch_iters)
+    finally:
+        if schedules:
+            tuner.install([])
     ref_fail = sum(0 if careful_test(e) else 1 for e, _, _ in ref)
-    mismatch = [primes[i] for i in range(len(primes)) if result[i][3] != ref[i][2]]
-    t_ref = sum((p - 2) * times[n] for p, n in zip(primes, nref)) * 1e-3
-    t_sel = sum((p - 2) * times[n] for p, n in zip(primes, accepted)) * 1e-3 + wasted_ms * 1e-3
-    shorter = sum(1 for a, b in zip(accepted, nref) if a < b)
+    mism_t = [p for p, r, q in zip(primes, t_res, ref) if r[3] != q[2]]
+    mism_b = [p for p, r, q in zip(primes, b_res, ref) if r[3] != q[2]]
+
+    def hours(lengths, waste_ms=0.0):
+        return (sum((p - 2) * times[n] for p, n in zip(primes, lengths)) + waste_ms) * 1e-3 / 3600.0
+
+    h_ref, h_tuned, h_timed = hours(nref), hours(ntuned), hours(t_acc, t_waste)
+    mean_iters = float(np.mean([p - 2 for p in primes]))
+    eph = lambda ms: 3600.0 * len(primes) / (ms * 1e-3 * mean_iters)
+    plain_ref_eph = len(primes) / h_ref
+    if log:
+        log("batched iteration: reference lengths %.4f ms, selected %.4f ms" % (b_ref, b_sel))
     return {
         "exponents": len(primes),
         "range": [min(primes), max(primes)],
         "judge": "careful round-off test: %d iterations, max <= %.2f and mean <= %.2f" % (judge_iters, LIMIT, LIMIT_AVG),
-        "selected_shorter_than_reference": shorter,
-        "selected_same_as_reference": sum(1 for a, b in zip(accepted, nref) if a == b),
-        "selected_longer_than_reference": sum(1 for a, b in zip(accepted, nref) if a > b),
-        "fallbacks": int(sum(fallbacks)),
-        "exponents_with_fallback": int(sum(1 for f in fallbacks if f)),
         "reference_length_judge_failures": ref_fail,
-        "residue_mismatches_vs_reference_length": mismatch,
-        "max_roundoff_selected": max(r[0] for r in result),
-        "projected_hours_reference_policy": t_ref / 3600.0,
-        "projected_hours_selector": t_sel / 3600.0,
-        "time_saved_fraction": 1.0 - t_sel / t_ref,
-        "fallback_cost_hours": wasted_ms * 1e-3 / 3600.0,
+        "timed": {
+            "shorter_than_reference": sum(a < b for a, b in zip(t_acc, nref)),
+            "longer_than_reference": sum(a > b for a, b in zip(t_acc, nref)),
+            "fallbacks": int(sum(t_fb)),
+            "residue_mismatches_vs_reference_length": mism_t,
+            "max_roundoff": max(r[0] for r in t_res),
+            "projected_hours": h_timed,
+        },
+        "batched": {
+            "cost_table": "batched per-test ms" if batch_times else "none (shortest safe length)",
+            "shorter_than_reference": sum(a < b for a, b in zip(b_acc, nref)),
+            "longer_than_reference": sum(a > b for a, b in zip(b_acc, nref)),
+            "fallbacks": int(sum(b_fb)),
+            "residue_mismatches_vs_reference_length": mism_b,
+            "max_roundoff": max(r[0] for r in b_res),
+            "words_fraction_of_reference": float(sum(b_acc)) / float(sum(nref)),
+            "schedules": "tuned batched table" if schedules else "built-in preference",
+        },
+        "plain_projected_hours": {"reference_policy": h_ref, "reference_policy_tuned_list": h_tuned,
+                                  "selector_timed": h_timed},
+        "time_saved_fraction_plain": 1.0 - h_timed / h_ref,
+        "time_saved_fraction_plain_vs_tuned_list": 1.0 - h_timed / h_tuned,
+        "batched_iteration_ms": {"reference_lengths": b_ref, "selected_lengths": b_sel},
+        "exps_per_hour": {"plain_reference_lengths": plain_ref_eph, "plain_selector": len(primes) / h_timed,
+                          "batched_reference_lengths": eph(b_ref), "batched_selected_lengths": eph(b_sel)},
+        "time_saved_fraction_batched": 1.0 - b_sel / b_ref,
+        "batched_selected_vs_plain_reference": eph(b_sel) / plain_ref_eph,
         "per_exponent": [
-            {"p": p, "n_reference": a, "n_first": b, "n_final": c, "fallbacks": f, "max_err": r[0],
-             "mean_err": r[1], "res64": "%016x" % r[2]}
-            for p, a, b, c, f, r in zip(primes, nref, first, accepted, fallbacks, result)
+            {"p": p, "offset": o, "n_reference": a, "n_tuned_list": tl, "n_timed_first": tf, "n_timed": ta,
+             "n_batched_first": bf, "n_batched": ba, "fallbacks_timed": fa, "fallbacks_batched": fb,
+             "max_err_timed": rt[0], "max_err_batched": rb[0], "res64": "%016x" % rt[2]}
+            for p, o, a, tl, tf, ta, bf, ba, fa, fb, rt, rb in zip(primes, offs, nref, ntuned, t_first, t_acc, b_first,
+                                                                  b_acc, t_fb, b_fb, t_res, b_res)
         ],
     }
 
@@ -472,6 +588,8 @@ def main(argv=None):
     a = sub.add_parser("evaluate")
     a.add_argument("--model", default=MODEL_PATH)
     a.add_argument("--times", default=TIMES_PATH)
+    a.add_argument("--batch-times", default=BATCH_TIMES_PATH)
+    a.add_argument("--schedules", default=None)
     a.add_argument("--count", type=int, default=W.C5_EVAL_COUNT)
     a.add_argument("--out")
     args = ap.parse_args(argv)
@@ -497,8 +615,12 @@ def main(argv=None):
     elif args.cmd == "evaluate":
         times = read_times(args.times)
         primes = W.log_uniform_primes(W.C5_EVAL_SEED, args.count, *W.C5_EVAL_RANGE)
-        with Selector(args.model, times=times) as s:
-            rep = evaluate(s, primes, times, log=_log)
+        from . import tuner
+
+        btimes = read_times(args.batch_times) if args.batch_times and os.path.exists(args.batch_times) else None
+        sched = args.schedules or tuner.BATCH_TABLE
+        sched = tuner.load(sched) if os.path.exists(sched) else None
+        rep = evaluate(args.model, primes, times, batch_times=btimes, schedules=sched, log=_log)
         summary = {k: v for k, v in rep.items() if k != "per_exponent"}
         _log(json.dumps(summary))
         if args.out:

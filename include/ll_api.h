@@ -187,18 +187,23 @@ int ll_selector_candidates(uint32_t p, uint32_t *ns, uint32_t capacity);
 int ll_selector_load(int32_t device, const char *path, ll_selector **out);
 void ll_selector_destroy(ll_selector *sel);
 const char *ll_selector_last_error(const ll_selector *sel);
-/* Measured ms/iteration per length (the fft bench table); with a table the
- * selector takes the fastest acceptable length, without one the shortest. */
+/* Measured ms/iteration per length (the fft bench table, or per-test cost
+ * in the batched engine); with a table the selector takes the fastest of the
+ * lengths predicted safe in [0.62, 1.25] x the reference policy's and the
+ * longer ones up to 2 x (safe without a prediction); without one, the
+ * shortest length predicted safe no longer than the reference policy's. */
 int ll_selector_set_times(ll_selector *sel, uint32_t count, const uint32_t *ns, const float *ms);
-/* log2 of the predicted full-test maximum roundoff at each length. */
+/* log2 of the predicted full-test maximum roundoff at each length: the
+ * network's window prediction + growth_scale x the Gaussian growth term +
+ * margin (growth_scale and margin are calibrated on held-out traces). */
 int ll_selector_predict(ll_selector *sel, uint32_t p, uint32_t count, const uint32_t *ns,
                         float *log2_err);
 /* Replaces choose_fft_length (CUDALucas.cu:1027-1068) when a model is
- * loaded.  above = 0: the first choice, never longer than the reference
- * policy's length (which is returned if no shorter one is predicted safe).
- * above = n: the fallback after length n failed the round-off test
- * (CUDALucas.cu:2589-2648): the best acceptable length longer than n.
- * pred_err (may be NULL) receives the predicted full-test maximum. */
+ * loaded.  above = 0: the first choice (see ll_selector_set_times; the
+ * reference policy's length if none is predicted safe).  above = n: the
+ * fallback after length n failed the round-off test (CUDALucas.cu:2589-2648):
+ * the best acceptable length longer than n.  pred_err (may be NULL)
+ * receives the predicted full-test maximum. */
 int ll_selector_choose(ll_selector *sel, uint32_t p, uint32_t above, float *pred_err);
 
 /* ---- host codec (CUDALucas.cu:1280-1342, 1465-1614) ------------------ */

@@ -23,6 +23,11 @@ from cudalucas_amd import workloads as W  # noqa: E402
 
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "gmp_residues.json")
 
+# exponents of the two-sweep schedule tests large enough that GMP takes
+# more than a few seconds for SWEEP_ITERS iterations
+SWEEP_ITERS = 200
+SWEEP_FIXTURE_PRIMES = [9914521, 10616897, 19535569, 29097221, 48495761, 67895203]
+
 
 def entry(p, iters, words):
     return {
@@ -45,6 +50,9 @@ def main():
         jobs.append((W.c3_prime(n), [200]))
     # large single exponent: checkpoints every 100 iterations up to 1000
     jobs.append((W.C2_P, list(range(100, W.C2_CHECK_ITER + 1, 100))))
+    # two-sweep schedule cases of tests/test_gpu_parity.py above 5e6
+    for p in SWEEP_FIXTURE_PRIMES:
+        jobs.append((p, [SWEEP_ITERS]))
     for p, marks in jobs:
         key = str(p)
         have = {e["iterations"] for e in data.get(key, [])}

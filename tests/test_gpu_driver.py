@@ -160,3 +160,49 @@ def test_sigint_writes_checkpoint(tmp_path):
     unshifted = ((value >> off) | (value << (p - off))) & ((1 << p) - 1)
     w, _, _ = L.run_ll(p, 131072, nxt - 1, offset=3)
     assert O.int_to_words(unshifted, p).tolist() == w.tolist()
+
+
+def test_cufftbench_and_threadbench_write_tuning_files(tmp_path):
+    """-cufftbench (CUDALucas.cu:2413-2587) and -threadbench
+    (CUDALucas.cu:2143-2276) write "<device> fft.txt" / "<device>
+    schedules.txt"; a later run reads both back."""
+    r = drive(["-cufftbench", 16, 40, 1], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    fft = [f for f in os.listdir(tmp_path) if f.endswith(" fft.txt")]
+    assert len(fft) == 1
+    rows = [l.split() for l in (tmp_path / fft[0]).read_text().splitlines() if l[:5].strip().isdigit()]
+    assert rows and all(L.fft_supported(int(k) * 1024) and float(ms) > 0 for k, _, ms in rows)
+    assert "16" in [k for k, _, _ in rows]  # powers of two are always kept
+    r = drive(["-threadbench", 256, 256, 1, 8], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    sch = [f for f in os.listdir(tmp_path) if f.endswith(" schedules.txt")]
+    assert len(sch) == 1
+    k, path, w, h, rr, ms = (tmp_path / sch[0]).read_text().split()[:6]
+    assert (int(k), int(path)) == (256, 1) and int(w) * int(h) * 2 == 262144
+    # the files are read back: a run still matches GMP
+    p = 1257787
+    r = drive(["-x", "1000", "--stop-after", 2000, "--offset", 5, p], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    rep = re.findall(r"Iteration (\d+) / %d, 0x([0-9a-f]{16})" % p, r.stdout)
+    assert rep and int(rep[0][1], 16) == gmp_res64(p, int(rep[0][0]))
+
+
+def test_info_prints_device(tmp_path):
+    r = drive(["-info"], tmp_path)  # no exponent and no worktodo: exits 2 after the listing
+    assert "Device Name" in r.stdout and "multiProcessorCount" in r.stdout, r.stdout + r.stderr
+
+
+def test_learned_selector_run_matches_gmp(tmp_path):
+    from cudalucas_amd import selector as S
+
+    if not os.path.exists(S.MODEL_PATH):
+        pytest.skip("no trained model committed")
+    p = 2000003
+    r = drive(["--selector", S.MODEL_PATH, "--selector-times", S.TIMES_PATH, "-x", "1000", "--stop-after", 3000,
+               "--offset", 17, p], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Learned length selection:" in r.stdout
+    rep = re.findall(r"Iteration (\d+) / %d, 0x([0-9a-f]{16})" % p, r.stdout)
+    assert rep
+    for it, res in rep:
+        assert int(res, 16) == gmp_res64(p, int(it))

@@ -103,12 +103,16 @@ SWEEP_CASES = [
     (29097221, 1572864, 768, 1024, 8),   # 3 * 2^18 complex points
     (48495761, 2621440, 640, 2048, 8),   # 5 * 2^18
     (67895203, 3670016, 896, 2048, 8),   # 7 * 2^18
+    (4723783, 262144, 512, 256, 16),     # 16-line carry workgroups (256-byte column runs)
+    (10616897, 589824, 72, 4096, 8),     # 4096-point columns, 512-thread square workgroups
 ]
 
 
 @pytest.mark.parametrize("p,n,w,h,r", SWEEP_CASES)
-def test_sweeps_vs_gmp(p, n, w, h, r):
-    iters = 120
+def test_sweeps_vs_gmp(p, n, w, h, r, gmp_fixtures):
+    """200 iterations of every schedule against GMP: live for small p, the
+    committed GMP fixture (Res64 + SHA-256 of the residue) above 5e6."""
+    iters = 200
     announce("sweeps p=%d n=%d W=%d H=%d R=%d" % (p, n, w, h, r))
     with L.LLContext(p, n, width=w, height=h, lines=r, flags=L.LL_OPT_FORCE_SWEEPS) as c:
         info = c.info()
@@ -116,8 +120,12 @@ def test_sweeps_vs_gmp(p, n, w, h, r):
         c.load_initial(4242)
         m = c.iterate(iters)
         words, res = c.residue()
-    g = O.gmp_ll(p, iters)
-    assert (words == g).all()
+    fx = gmp_fixtures.get((p, iters))
+    if fx is not None:
+        assert "0x%016x" % res == fx["res64"] and sha(words) == fx["sha256"]
+    else:
+        assert p < 5_000_000, "missing GMP fixture for p=%d" % p
+        assert (words == O.gmp_ll(p, iters)).all()
     assert m < ROUNDOFF_LIMIT
 
 

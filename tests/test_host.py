@@ -216,3 +216,58 @@ def test_driver_ini_precedence(tmp_path):
     # intervals are rounded to k * 10^m, k in {1, 2, 5}
     r = run_driver(["-x", "3000", "--print-config"], tmp_path)
     assert "report=2000" in r.stdout
+
+
+def test_fft_length_list_can_be_replaced_and_restored():
+    """init_ffts merging "<device> fft.txt" (CUDALucas.cu:947-1025) goes
+    through ll_fft_length_list_set; choose_fft_length then works on it."""
+    lib = L.lib()
+    default = L.fft_lengths()
+    custom = np.array([1024 * k for k in (1, 2, 4, 8, 20, 64, 128, 256, 512, 1024, 4096, 8192)], dtype=np.uint32)
+    assert lib.ll_fft_length_list_set(len(custom), custom.ctypes.data) == 0
+    try:
+        assert L.fft_lengths() == [int(x) for x in custom]
+        n, idx = L.choose_fft_length(500009)  # lb ~ 24.5K words: next listed length is 64K
+        assert n == 65536 and L.fft_lengths()[idx] == n
+        bad = np.array([2048, 1024], dtype=np.uint32)
+        assert lib.ll_fft_length_list_set(2, bad.ctypes.data) == L.LL_ERR_ARG  # not ascending
+    finally:
+        assert lib.ll_fft_length_list_set(0, None) == 0
+    assert L.fft_lengths() == default
+
+
+def test_schedule_table_overrides_builtin_preference():
+    from cudalucas_amd import tuner as T
+
+    n = 262144
+    every = T.schedules(n)
+    assert len(every) >= 4 and all(s.n == n for s in every)
+    default = L.fft_schedule(n)
+    other = next(s for s in every if (s.width, s.height, s.lines) != default[1:])
+    T.install([other])
+    try:
+        assert L.fft_schedule(n) == ("sweeps", other.width, other.height, other.lines)
+        # an explicit schedule in the options still wins
+        assert L.fft_schedule(n, width=default[1], height=default[2], lines=default[3]) == default
+        assert L.fft_schedule(524288) == L.fft_schedule(524288)  # other lengths untouched
+    finally:
+        T.install([])
+    assert L.fft_schedule(n) == default
+
+
+def test_tuner_table_round_trip(tmp_path):
+    from cudalucas_amd import tuner as T
+
+    t = [T.Schedule(8192, 0, 4096, 1, 1, 0.01), T.Schedule(131072, 1, 256, 256, 8, 0.02)]
+    T.save(t, str(tmp_path / "s.txt"), "test")
+    back = T.load(str(tmp_path / "s.txt"))
+    assert [b.key() for b in back] == [x.key() for x in t]
+    assert T.parse_lengths("4096K,5M") == [4194304, 5242880]
+    assert all(L.fft_supported(n) for n in T.parse_lengths("8K-64K"))
+
+
+def test_driver_bench_options_parse(tmp_path):
+    r = run_driver(["-h"], tmp_path)
+    assert "-cufftbench" in r.stderr and "-threadbench" in r.stderr and "--selector" in r.stderr
+    r = run_driver(["-cufftbench", "1"], tmp_path)
+    assert r.returncode == 2 and "can't parse -cufftbench" in r.stderr
