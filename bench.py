@@ -158,8 +158,15 @@ def main():
         print(json.dumps({"copy_bytes": copy_bytes, "copy_ms": copy_ms, "copy_gbs": round(copy_gbs, 1)}))
         return
 
+    from cudalucas_amd import tuner as T
+
     p, n = W.C2_P, W.C2_N
+    single_tuned = os.path.exists(T.SINGLE_TABLE)
+    if single_tuned:  # schedules picked by the single-exponent autotuner
+        T.install(T.load(T.SINGLE_TABLE))
     ctx = L.LLContext(p, n)
+    if single_tuned:
+        T.install([])
     info = ctx.info()
     ctx.load_initial(0)
     if args.warmup:
@@ -209,7 +216,8 @@ def main():
             "(%.2f bits/word), s_0=4" % (p, n, p / n),
             "p": p,
             "n": n,
-            "schedule": "%s W=%d H=%d R=%d" % (info["path"], info["width"], info["height"], info["lines"]),
+            "schedule": "%s W=%d H=%d R=%d (%s)" % (info["path"], info["width"], info["height"], info["lines"],
+                                                     "autotuned" if single_tuned else "built-in preference"),
         },
         "gpu_ms_per_iter": round(gpu_ms_per_iter, 5),
         "iter_per_s": round(1000.0 / ms_per_step, 1),

@@ -13,6 +13,24 @@
 
 namespace llamd {
 
+// Loads and stores through global-address-space pointers: the signal and
+// table pointers arrive in structs as generic pointers, which would
+// otherwise compile to flat memory instructions (slower to issue, and
+// counted against the LDS wait counter too).
+typedef double gd2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ cplx gload(const cplx *p) {
+  const gd2 v = *(const __attribute__((address_space(1))) gd2 *)p;
+  return {v.x, v.y};
+}
+__device__ __forceinline__ void gstore(cplx *p, cplx c) {
+  const gd2 v = {c.x, c.y};
+  *(__attribute__((address_space(1))) gd2 *)p = v;
+}
+template <typename V>
+__device__ __forceinline__ V gload(const V *p) {
+  return *(const __attribute__((address_space(1))) V *)p;
+}
+
 __device__ __forceinline__ cplx cadd(cplx a, cplx b) { return {a.x + b.x, a.y + b.y}; }
 __device__ __forceinline__ cplx csub(cplx a, cplx b) { return {a.x - b.x, a.y - b.y}; }
 __device__ __forceinline__ cplx cconj(cplx a) { return {a.x, -a.y}; }
@@ -259,7 +277,7 @@ struct QuarterRoots {
 // Copies the quarter-wave table of order L from global memory into LDS.
 template <int L, int T>
 __device__ __forceinline__ void load_quarter(double *dst, const double *__restrict__ src) {
-  for (int i = threadIdx.x; i <= L / 4; i += T) dst[i] = src[i];
+  for (int i = threadIdx.x; i <= L / 4; i += T) dst[i] = gload(src + i);
 }
 
 // One Stockham pass over NB transforms of length L held at buf[b*LP + lpad(i)].

@@ -16,11 +16,16 @@ struct TwoLevel {
   uint32_t mask;
 };
 
-// Device-resident iteration state of the two-sweep path.
+// Device-resident iteration state of the two-sweep path.  Workgroup 0 of
+// the square sweep opens each iteration (epoch, offset, position of the -2,
+// whether a batched test still runs); the carry sweep of the same iteration
+// only reads it, so no atomics are needed.
 struct DevState {
-  unsigned long long ticket;  // carry-sweep admission counter (monotonic)
-  uint32_t done;              // carry-sweep workgroups finished this launch
-  uint32_t offset;            // shift offset of the stored value
+  uint32_t epoch;      // iterations opened so far: the carry shuttle's tag
+  uint32_t offset;     // shift offset of the value once this iteration ends
+  uint32_t run;        // 0: a finished batched test (the carry sweep idles)
+  uint32_t m2_digit;   // digit that takes the -2 in this iteration
+  long long m2_value;  // 2^(bit position inside that digit)
 };
 
 struct PlanArgs {

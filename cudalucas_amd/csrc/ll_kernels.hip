@@ -7,27 +7,9 @@
 //    signal lives in one workgroup's LDS and the kernel loops over many
 //    iterations without touching HBM (batched: one workgroup per exponent).
 //
-//  * two sweeps over HBM for large N, M = W x H.  Complex element
-//    m = a*H + b holds words (x[2m], x[2m+1]); lines b are tiled by R, so the
-//    element lives at ((b / R) W + a) R + (b mod R): a carry workgroup's R
-//    lines are one contiguous block, a column is read in runs of R.
-//      square_sweep : per column pair (k1, W-k1): forward H-point DFT,
-//                     real-pairing squaring, inverse H-point DFT.
-//      carry_sweep  : per block of R lines b: inverse twiddle, inverse W-point
-//                     DFT, unweight/round/-2/balanced carry, carry look-back
-//                     from the previous block, reweight, forward W-point
-//                     DFT, forward twiddle.
-//    Each sweep reads and writes the signal once: 32 N bytes per iteration.
-//
-// Carries between line blocks travel through an epoch-tagged shuttle
-// (64-bit words: iteration epoch in the high half, int32 carry in the low
-// half) written and polled with agent-scope atomics, so no fence or flag is
-// needed.  Blocks take tickets from an atomic counter, so a block only waits
-// on blocks that were admitted before it.  The cyclic wrap (the last block's
-// carries feed block 0) is handled by admitting H/R + 1 workgroups per
-// launch: ticket 0 computes block 0 only to publish its carries; the last
-// ticket recomputes block 0 from the untouched input, applies the wrapped
-// carries and writes it.
+//  * two sweeps over HBM for large N (ll_sweeps.h): square_sweep over
+//    column pairs, carry_sweep over blocks of lines, carries handed between
+//    blocks through an epoch-tagged shuttle.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 

@@ -24,11 +24,13 @@
 // Carries between line blocks travel through an epoch-tagged shuttle
 // (64-bit words: iteration epoch in the high half, int32 carry in the low
 // half) written and polled with agent-scope atomics, so no fence or flag is
-// needed.  Blocks take tickets from an atomic counter, so a block only waits
-// on blocks that were admitted before it.  The cyclic wrap (the last block's
-// carries feed block 0, one column to the left) is the one wait on a later
-// ticket; it cannot deadlock because every block publishes its own carries
-// before it waits, so the last block never waits on block 0.
+// needed.  Block k waits only on block k-1, dispatched before it; the
+// cyclic wrap (the last block's carries feed block 0, one column to the
+// left) is the one wait on a later block, and it cannot deadlock because
+// every block publishes its own carries before it waits, so the last block
+// never waits on block 0.  The square sweep's workgroup 0 opens each
+// iteration in a small device state (epoch, offset, -2 position) that the
+// carry sweep reads.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -45,8 +47,8 @@ constexpr uint32_t kSpinLimit = 1u << 22;
 // ------------------------------------------------------------------ helpers
 
 __device__ __forceinline__ cplx twiddle_m(const TwoLevel &t, uint32_t e) {
-  const cplx h = t.hi[e >> t.shift];
-  const cplx l = t.lo[e & t.mask];  // (cos - 1, sin) of a small angle
+  const cplx h = gload(t.hi + (e >> t.shift));
+  const cplx l = gload(t.lo + (e & t.mask));  // (cos - 1, sin) of a small angle
   return {h.x + fma(h.x, l.x, -h.y * l.y), h.y + fma(h.x, l.y, h.y * l.x)};
 }
 
@@ -150,7 +152,22 @@ __device__ __forceinline__ void pair_square(cplx &A, cplx &B, cplx t) {
 template <int H, int T>
 __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict__ args) {
   const PlanArgs &A = args[blockIdx.y];
-  if (A.remaining && *A.remaining == 0) return;  // this test has finished
+  const bool finished = A.remaining && *A.remaining == 0;  // a batched test that is done
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    // open the iteration for the carry sweep that follows this launch
+    DevState &st = *A.st;
+    if (finished) {
+      st.run = 0;
+    } else {
+      const Minus2 m2 = minus2_position(st.offset, A.p, A.n);
+      st.m2_digit = m2.digit;
+      st.m2_value = m2.value;
+      st.offset = m2.new_offset;
+      st.epoch += 1;
+      st.run = 1;
+    }
+  }
+  if (finished) return;
   constexpr int LP = lpitch(H);
   __shared__ cplx lds[2 * LP];
   __shared__ double qc[H / 4 + 1];
@@ -168,17 +185,27 @@ __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict_
   // is applied here on the way in and undone on the way out.  Element
   // b = tid + s T of column c takes w^(c tid) * w^(c s T): two table roots
   // per thread and column instead of one per element.
+  // All of a thread's loads are issued before the first is used, so each
+  // wave keeps 2 H / T loads in flight.
   static_assert(H % T == 0, "square sweep: H must be a multiple of T");
   {
-    const cplx t0 = twiddle_m(A.twM, c0 * (uint32_t)tid), t1 = twiddle_m(A.twM, c1 * (uint32_t)tid);
-#pragma unroll 2
-    for (int st = 0; st < H / T; st++) {
+    constexpr int NE = H / T;
+    cplx v0[NE], v1[NE];
+#pragma unroll
+    for (int st = 0; st < NE; st++) {
       const int i = tid + st * T;
       const size_t off = ((size_t)(i >> rs) * W << rs) | (i & rmask);
+      v0[st] = gload(col0 + off);
+      v1[st] = gload(col1 + off);
+    }
+    const cplx t0 = twiddle_m(A.twM, c0 * (uint32_t)tid), t1 = twiddle_m(A.twM, c1 * (uint32_t)tid);
+#pragma unroll
+    for (int st = 0; st < NE; st++) {
+      const int i = tid + st * T;
       const cplx w0 = cmul(t0, twiddle_m(A.twM, c0 * (uint32_t)(st * T)));
       const cplx w1 = cmul(t1, twiddle_m(A.twM, c1 * (uint32_t)(st * T)));
-      lds[lpad(i)] = cmul(col0[off], w0);
-      lds[LP + lpad(i)] = cmul(col1[off], w1);
+      lds[lpad(i)] = cmul(v0[st], w0);
+      lds[LP + lpad(i)] = cmul(v1[st], w1);
     }
   }
   __syncthreads();
@@ -223,23 +250,24 @@ __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict_
       const size_t off = ((size_t)(i >> rs) * W << rs) | (i & rmask);
       const cplx w0 = cmul(t0, twiddle_m(A.twM, c0 * (uint32_t)(st * T)));
       const cplx w1 = cmul(t1, twiddle_m(A.twM, c1 * (uint32_t)(st * T)));
-      out0[off] = cmulc(lds[lpad(i)], w0);
-      out1[off] = cmulc(lds[LP + lpad(i)], w1);
+      gstore(out0 + off, cmulc(lds[lpad(i)], w0));
+      gstore(out1 + off, cmulc(lds[LP + lpad(i)], w1));
     }
   }
 }
 
 // ------------------------------------------------------------- carry sweep
-// H/R workgroups, one per block of R lines, admitted in ticket order.  Each
-// publishes the carries out of its block before waiting for those into it,
-// so every wait is on a block admitted earlier -- except block 0, which
-// waits for the last block's carries (the cyclic wrap 2^p = 1) after
-// publishing its own; no block waits on block 0 finishing, so this cannot
-// deadlock whatever the number of co-resident workgroups.
+// H/R workgroups, workgroup k owning block k of R lines.  Each publishes the
+// carries out of its block before it waits for those into it, so a wait is
+// on a lower-numbered block, which the hardware dispatches first -- except
+// block 0, which waits for the last block's carries (the cyclic wrap
+// 2^p = 1) after publishing its own; no block waits on block 0 finishing.
+// Should a wait ever outlast kSpinLimit polls, the iteration is reported as
+// failed (error 3.0) rather than hanging the device.
 template <int W, int R, int T>
 __global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArgs *__restrict__ args) {
   const PlanArgs &A = args[blockIdx.y];
-  if (A.remaining && *A.remaining == 0) return;  // finished: take no ticket
+  if (!A.st->run) return;  // a finished batched test
   constexpr int LP = lpitch(W);
   constexpr int CPT = (W + T - 1) / T;  // columns per thread
   __shared__ cplx lds[R * LP];
@@ -247,35 +275,40 @@ __global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArg
   __shared__ int fb[2 * R];  // weight factors shared by every column
   __shared__ double wb[2 * R], iwb[2 * R];
   __shared__ float red[T / 64];
-  __shared__ uint64_t s_ticket;
-  __shared__ Minus2 s_m2;
   const int tid = threadIdx.x;
   const uint32_t H = A.H;
   const uint32_t G = H / R;
+  const uint32_t epoch = A.st->epoch;
+  const uint32_t m2_digit = A.st->m2_digit;
+  const double dmod = (double)A.st->m2_value;
+  const uint32_t blk = blockIdx.x;
 
-  if (tid == 0) {
-    s_ticket = atomicAdd(&A.st->ticket, 1ull);
-    s_m2 = minus2_position(A.st->offset, A.p, A.n);
-  }
   load_quarter<W, T>(qc, A.qcW);
   __syncthreads();
-  const uint64_t tk = s_ticket;
-  const uint32_t epoch = (uint32_t)(tk / G);
-  const uint32_t blk = (uint32_t)(tk % G);
   if (tid < 2 * R) {
     const uint32_t i = 2 * blk * R + tid;
-    fb[tid] = A.fB[i];
-    wb[tid] = A.wB[i];
-    iwb[tid] = A.iwB[i] * A.scale;
+    fb[tid] = gload(A.fB + i);
+    wb[tid] = gload(A.wB + i);
+    iwb[tid] = gload(A.iwB + i) * A.scale;
   }
   const uint32_t b0 = blk * R;
-  const Minus2 m2 = s_m2;
   const QuarterRoots<W> tw{qc};
 
-  // load R lines (the square sweep has already undone the column twiddle)
-  for (int e = tid; e < R * W; e += T) {
-    const int r = e % R, k1 = e / R;
-    lds[r * LP + lpad(k1)] = A.x[(size_t)b0 * W + e];
+  // load R lines (the square sweep has already undone the column twiddle),
+  // every load of the thread in flight before the first LDS write
+  {
+    constexpr int NE = (R * W + T - 1) / T;
+    cplx v[NE];
+#pragma unroll
+    for (int s = 0; s < NE; s++) {
+      const int e = tid + s * T;
+      if ((R * W) % T == 0 || e < R * W) v[s] = gload(A.x + (size_t)b0 * W + e);
+    }
+#pragma unroll
+    for (int s = 0; s < NE; s++) {
+      const int e = tid + s * T;
+      if ((R * W) % T == 0 || e < R * W) lds[(e % R) * LP + lpad(e / R)] = v[s];
+    }
   }
   __syncthreads();
   fft_lds<W, R, T, true>(lds, tw);
@@ -284,11 +317,10 @@ __global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArg
   // in exact FP64 (balance_f64).  The balanced digits go back into the
   // column's own LDS slots until the carry-in arrives.
   const double base_lo = A.base_lo, ibase_lo = 1.0 / base_lo;  // 2^q and 2^-q
-  const double dmod = (double)m2.value;
   double emax = 0.0;
   for (int a = tid; a < W; a += T) {
-    const int fa = A.fA[a];
-    const double iwa = A.iwA[a];
+    const int fa = gload(A.fA + a);
+    const double iwa = gload(A.iwA + a);
     const uint32_t j0 = 2u * ((uint32_t)a * H + b0);
     double carry = 0.0;
 #pragma unroll
@@ -305,7 +337,7 @@ __global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArg
         const double t = (e ? v.y : v.x) * (iwa * iwb[i] * (wrap ? 2.0 : 1.0));
         double rt = rint(t);
         emax = fmax(emax, fabs(t - rt));
-        if (j0 + i == m2.digit) rt -= dmod;
+        if (j0 + i == m2_digit) rt -= dmod;
         d[e] = balance_f64(rt + carry, big ? 2.0 * base_lo : base_lo, big ? 0.5 * ibase_lo : ibase_lo, carry);
       }
       lds[r * LP + lpad(a)] = {d[0], d[1]};
@@ -339,8 +371,8 @@ __global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArg
     // through the whole chain; anything left after 2R words (0 in
     // practice) stays in the last word.
     double cc = (double)(int32_t)(uint32_t)word;
-    const int fa = A.fA[a];
-    const double wa = A.wA[a];
+    const int fa = gload(A.fA + a);
+    const double wa = gload(A.wA + a);
 #pragma unroll
     for (int r = 0; r < R; r++) {
       const cplx dv = lds[r * LP + lpad(a)];
@@ -362,19 +394,29 @@ __global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArg
   }
   __syncthreads();
   fft_lds<W, R, T, false>(lds, tw);
-  for (int e = tid; e < R * W; e += T) {
-    const int r = e % R, k1 = e / R;
-    A.x[(size_t)b0 * W + e] = lds[r * LP + lpad(k1)];
+  {
+    constexpr int NE = (R * W + T - 1) / T;
+    cplx v[NE];
+    // an opaque copy of tid keeps the compiler from holding the load
+    // phase's LDS addresses in registers across the whole kernel
+    int t2 = tid;
+    asm volatile("" : "+v"(t2));
+#pragma unroll
+    for (int s = 0; s < NE; s++) {
+      const int e = t2 + s * T;
+      if ((R * W) % T == 0 || e < R * W) v[s] = lds[(e % R) * LP + lpad(e / R)];
+    }
+#pragma unroll
+    for (int s = 0; s < NE; s++) {
+      const int e = tid + s * T;
+      if ((R * W) % T == 0 || e < R * W) gstore(A.x + (size_t)b0 * W + e, v[s]);
+    }
   }
   const float m = block_max<T>((float)emax, red);
   if (tid == 0) {
-    atomicMax(&A.err[epoch & A.err_mask], __float_as_uint(m));
-    const uint32_t old = atomicAdd(&A.st->done, 1u);
-    if (old == G - 1) {
-      A.st->done = 0;
-      A.st->offset = m2.new_offset;
-      if (A.remaining) *A.remaining -= 1;
-    }
+    atomicMax(&A.err[(epoch - 1) & A.err_mask], __float_as_uint(m));
+    // no carry workgroup reads `remaining`: the next square sweep does
+    if (blk == 0 && A.remaining) *A.remaining -= 1;
   }
 }
 

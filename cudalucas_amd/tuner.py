@@ -31,6 +31,7 @@ from . import workloads as W
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 BATCH_TABLE = os.path.join(_HERE, "data", "mi355x_schedules_batched.txt")
+SINGLE_TABLE = os.path.join(_HERE, "data", "mi355x_schedules_single.txt")
 
 
 class Schedule(ctypes.Structure):
