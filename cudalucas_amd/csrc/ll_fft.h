@@ -335,6 +335,93 @@ __device__ __forceinline__ void fft_lds(cplx *buf, const TW &tw) {
   }
 }
 
+// ------------------------------------------------------ wave-local passes
+// When L = RAD^k, every pass has the same Q = L/RAD butterflies per line and
+// butterfly u = tid + s T works in line u / Q.  If Q divides 64 (or the
+// workgroup is one wave) all butterflies of a line belong to one wave in
+// every pass, so the passes need no workgroup barrier: LDS instructions of
+// one wave execute in order, and a wavefront fence keeps the compiler from
+// moving them across a pass boundary.
+constexpr int uniform_radix(int L) {
+  int r = 16;
+  for (; r >= 8; r /= 2) {
+    int x = L;
+    while (x % r == 0) x /= r;
+    if (x == 1) return r;
+  }
+  return 0;
+}
+
+template <int L, int NB, int T>
+constexpr bool wave_local_ok() {
+  constexpr int RAD = uniform_radix(L);
+  constexpr int Q = RAD ? L / RAD : 0;
+  return RAD != 0 && ((Q <= 64 && 64 % Q == 0) || T == 64) && T % 64 == 0 && (NB * Q) % T == 0;
+}
+
+template <int L, int RAD, int NS, int NB, int T, bool INV, typename TW>
+__device__ __forceinline__ void stockham_pass_wave(cplx *buf, const TW &tw) {
+  constexpr int Q = L / RAD;
+  constexpr int PER = NB * Q / T;
+  constexpr int LP = lpitch(L);
+  cplx v[PER][RAD];
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int s = 0; s < PER; s++) {
+    const int u = tid + s * T;
+    const int b = u / Q, j = u % Q;
+    const int k = j % NS;
+#pragma unroll
+    for (int i = 0; i < RAD; i++) {
+      cplx a = buf[b * LP + lpad(j + i * Q)];
+      if (NS > 1 && i > 0) {
+        const cplx w = tw((i * k) * (L / (NS * RAD)));
+        a = INV ? cmulc(a, w) : cmul(a, w);
+      }
+      v[s][i] = a;
+    }
+  }
+  // every lane of this wave has read its inputs before any lane writes
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+#pragma unroll
+  for (int s = 0; s < PER; s++) {
+    const int u = tid + s * T;
+    const int b = u / Q, j = u % Q;
+    const int k = j % NS;
+    dft<RAD, INV>(v[s]);
+    const int base = (j / NS) * NS * RAD + k;
+#pragma unroll
+    for (int i = 0; i < RAD; i++) buf[b * LP + lpad(base + i * NS)] = v[s][i];
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// NB transforms of length L = RAD^k, each wave transforming its own lines;
+// callers synchronise the workgroup before and after.
+template <int L, int NB, int T, bool INV, int NS = 1, typename TW>
+__device__ __forceinline__ void fft_lds_wave(cplx *buf, const TW &tw) {
+  static_assert(wave_local_ok<L, NB, T>(), "fft_lds_wave: plan is not wave-local");
+  if constexpr (NS < L) {
+    constexpr int RAD = uniform_radix(L);
+    stockham_pass_wave<L, RAD, NS, NB, T, INV>(buf, tw);
+    fft_lds_wave<L, NB, T, INV, NS * RAD>(buf, tw);
+  }
+}
+
+// Either form, chosen at compile time.
+template <int L, int NB, int T, bool INV, typename TW>
+__device__ __forceinline__ void fft_lds_auto(cplx *buf, const TW &tw) {
+  if constexpr (wave_local_ok<L, NB, T>()) {
+    __syncthreads();
+    fft_lds_wave<L, NB, T, INV>(buf, tw);
+    __syncthreads();
+  } else {
+    fft_lds<L, NB, T, INV>(buf, tw);
+  }
+}
+
 // ------------------------------------------------ per-thread twiddle cache
 // In pass PASS of an NB x L Stockham transform run by T threads, thread t
 // always handles butterflies u = t + s*T, so its twiddles w^(i*(j mod NS))

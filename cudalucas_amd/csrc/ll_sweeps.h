@@ -311,7 +311,7 @@ __global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArg
     }
   }
   __syncthreads();
-  fft_lds<W, R, T, true>(lds, tw);
+  fft_lds_auto<W, R, T, true>(lds, tw);
 
   // Round, subtract, carry along the 2R consecutive words of each column,
   // in exact FP64 (balance_f64).  The balanced digits go back into the
@@ -393,7 +393,7 @@ __global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArg
     }
   }
   __syncthreads();
-  fft_lds<W, R, T, false>(lds, tw);
+  fft_lds_auto<W, R, T, false>(lds, tw);
   {
     constexpr int NE = (R * W + T - 1) / T;
     cplx v[NE];

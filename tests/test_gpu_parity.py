@@ -185,12 +185,16 @@ def test_reference_selftest_res64(p, res64, n):
 
 # ---------------------------------------------------- large configurations
 
-def test_c2_77m_vs_gmp_fixture(gmp_fixtures):
+@pytest.mark.parametrize("schedule", [None, (512, 4096, 8)])
+def test_c2_77m_vs_gmp_fixture(gmp_fixtures, schedule):
+    """The built-in schedule and the autotuned one (the bench's)."""
     p, n = W.C2_P, W.C2_N
     marks = sorted(k[1] for k in gmp_fixtures if k[0] == p)
     assert marks, "missing GMP fixture for p=%d" % p
-    announce("C2 p=%d n=%d marks=%s" % (p, n, marks))
-    with L.LLContext(p, n) as c:
+    announce("C2 p=%d n=%d marks=%s schedule=%s" % (p, n, marks, schedule))
+    kw = {} if schedule is None else dict(width=schedule[0], height=schedule[1], lines=schedule[2],
+                                           flags=L.LL_OPT_FORCE_SWEEPS)
+    with L.LLContext(p, n, **kw) as c:
         c.load_initial(0)
         done, worst = 0, 0.0
         for mk in marks:
