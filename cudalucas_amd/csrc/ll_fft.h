@@ -325,13 +325,13 @@ __device__ __forceinline__ void stockham_pass(cplx *buf, const TW &tw) {
   __syncthreads();
 }
 
-template <int L, int NB, int T, bool INV, int PASS = 0, int NS = 1, typename TW>
+template <int L, int NB, int T, bool INV, int PASS = 0, int NS = 1, int END = -1, typename TW>
 __device__ __forceinline__ void fft_lds(cplx *buf, const TW &tw) {
-  if constexpr (NS < L) {
+  if constexpr (NS < L && PASS != END) {
     constexpr int R = plan_radix(L, PASS);
     static_assert(R > 0, "bad plan");
     stockham_pass<L, R, NS, NB, T, INV>(buf, tw);
-    fft_lds<L, NB, T, INV, PASS + 1, NS * R>(buf, tw);
+    fft_lds<L, NB, T, INV, PASS + 1, NS * R, END>(buf, tw);
   }
 }
 
@@ -501,13 +501,33 @@ __device__ __forceinline__ void stockham_pass_cached(cplx *buf, const PT &pt) {
 }
 
 // fft_lds with cached twiddles (pt: the PassTwiddles of the same L, NB, T).
-template <int L, int NB, int T, bool INV, int PASS = 0, int NS = 1, typename PT>
+// Passes PASS .. END-1 are run (END = -1: to the last pass).
+template <int L, int NB, int T, bool INV, int PASS = 0, int NS = 1, int END = -1, typename PT>
 __device__ __forceinline__ void fft_lds_cached(cplx *buf, const PT &pt) {
-  if constexpr (NS < L) {
+  if constexpr (NS < L && PASS != END) {
     constexpr int R = plan_radix(L, PASS);
     stockham_pass_cached<L, R, NS, NB, T, INV>(buf, pt);
-    fft_lds_cached<L, NB, T, INV, PASS + 1, NS * R>(buf, pt.next);
+    fft_lds_cached<L, NB, T, INV, PASS + 1, NS * R, END>(buf, pt.next);
   }
+}
+
+// Number of passes of the plan of L, and the span NS before pass P.
+constexpr int plan_passes(int L) {
+  int n = 0, ns = 1;
+  while (ns < L) ns *= plan_radix(L, n++);
+  return n;
+}
+constexpr int plan_span(int L, int P) {
+  int ns = 1;
+  for (int i = 0; i < P; i++) ns *= plan_radix(L, i);
+  return ns;
+}
+
+// The PassTwiddles level of pass K.
+template <int K, typename PT>
+__device__ __forceinline__ const auto &twiddle_level(const PT &pt) {
+  if constexpr (K == 0) return pt;
+  else return twiddle_level<K - 1>(pt.next);
 }
 
 }  // namespace llamd

@@ -181,14 +181,52 @@ __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict_
   const cplx *col0 = A.x + ((size_t)c0 << rs);
   const cplx *col1 = A.x + ((size_t)c1 << rs);
   load_quarter<H, T>(qc, A.qcH);
+  const QuarterRoots<H> tw{qc};
   // The four-step twiddle w_M^(c b) between the line and the column DFTs
-  // is applied here on the way in and undone on the way out.  Element
-  // b = tid + s T of column c takes w^(c tid) * w^(c s T): two table roots
-  // per thread and column instead of one per element.
-  // All of a thread's loads are issued before the first is used, so each
-  // wave keeps 2 H / T loads in flight.
-  static_assert(H % T == 0, "square sweep: H must be a multiple of T");
-  {
+  // is applied on the way in and undone on the way out.
+  //
+  // Fused form (when the first and last passes give every thread whole
+  // butterflies): each thread loads the R0 inputs of its first-pass
+  // butterfly straight from HBM -- rows j + i Q0 of its column, twiddled by
+  // w^(c j) * w^(c Q0 i) -- and runs that pass in registers; the last pass
+  // of the inverse transform likewise writes straight to HBM.  That saves
+  // two LDS round trips and their barriers.
+  constexpr int R0 = plan_radix(H, 0), Q0 = H / R0;
+  constexpr int NP = plan_passes(H), RL = plan_radix(H, NP - 1), QL = H / RL;
+  // (4096-point columns with 512 threads run out of registers when fused)
+  constexpr bool FUSED = (2 * Q0) % T == 0 && (2 * QL) % T == 0 && NP >= 2 && H < 4096;
+  const uint32_t cols[2] = {c0, c1};
+  if constexpr (FUSED) {
+    constexpr int PER = 2 * Q0 / T;
+    cplx v[PER][R0];
+#pragma unroll
+    for (int s = 0; s < PER; s++) {
+      const int u = tid + s * T, b = u / Q0, j = u % Q0;
+      const cplx *col = b ? col1 : col0;
+#pragma unroll
+      for (int i = 0; i < R0; i++) {
+        const int row = j + i * Q0;
+        v[s][i] = gload(col + (((size_t)(row >> rs) * W << rs) | (row & rmask)));
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < PER; s++) {
+      const int u = tid + s * T, b = u / Q0, j = u % Q0;
+      const uint32_t c = cols[b];
+      const cplx tj = twiddle_m(A.twM, c * (uint32_t)j);
+#pragma unroll
+      for (int i = 0; i < R0; i++) {
+        const cplx w = i ? cmul(tj, twiddle_m(A.twM, c * (uint32_t)(i * Q0))) : tj;
+        v[s][i] = cmul(v[s][i], w);
+      }
+      dft<R0, false>(v[s]);
+#pragma unroll
+      for (int i = 0; i < R0; i++) lds[b * LP + lpad(j * R0 + i)] = v[s][i];
+    }
+  } else {
+    // Element b = tid + s T of column c takes w^(c tid) * w^(c s T).  All
+    // of a thread's loads are issued before the first is used.
+    static_assert(H % T == 0, "square sweep: H must be a multiple of T");
     constexpr int NE = H / T;
     cplx v0[NE], v1[NE];
 #pragma unroll
@@ -209,10 +247,18 @@ __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict_
     }
   }
   __syncthreads();
-  const QuarterRoots<H> tw{qc};
-  PassTwiddles<H, 2, T> ptw;
-  ptw.init(tw);
-  fft_lds_cached<H, 2, T, false>(lds, ptw);
+  // Per-thread twiddles are cached in registers for both transforms (the
+  // LDS-table form is kept for shapes where registers run short).
+  constexpr bool CACHE = true;
+  PassTwiddles<(CACHE ? H : 16), 2, (CACHE ? T : 64)> ptw;
+  if constexpr (CACHE) {
+    ptw.init(tw);
+    if constexpr (FUSED) fft_lds_cached<H, 2, T, false, 1, R0>(lds, ptw.next);
+    else fft_lds_cached<H, 2, T, false>(lds, ptw);
+  } else {
+    if constexpr (FUSED) fft_lds<H, 2, T, false, 1, R0>(lds, tw);
+    else fft_lds<H, 2, T, false>(lds, tw);
+  }
   if (g != 0) {
     for (int k2 = tid; k2 < H; k2 += T) {
       const int k2b = H - 1 - k2;
@@ -239,10 +285,46 @@ __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict_
     }
   }
   __syncthreads();
-  fft_lds_cached<H, 2, T, true>(lds, ptw);
   cplx *out0 = A.x + ((size_t)c0 << rs);
   cplx *out1 = A.x + ((size_t)c1 << rs);
-  {
+  if constexpr (FUSED) {
+    if constexpr (CACHE) fft_lds_cached<H, 2, T, true, 0, 1, NP - 1>(lds, ptw);
+    else fft_lds<H, 2, T, true, 0, 1, NP - 1>(lds, tw);
+    // last inverse pass: rows j + i QL of each column, straight to HBM
+    constexpr int NSL = plan_span(H, NP - 1);
+    static_assert(NSL == QL, "last pass spans the whole column");
+    constexpr int PER = 2 * QL / T;
+    cplx v[PER][RL];
+#pragma unroll
+    for (int s = 0; s < PER; s++) {
+      const int u = tid + s * T, b = u / QL, j = u % QL;
+#pragma unroll
+      for (int i = 0; i < RL; i++) {
+        cplx a = lds[b * LP + lpad(j + i * QL)];
+        if (i > 0) {
+          if constexpr (CACHE) a = cmulc(a, twiddle_level<NP - 1>(ptw).w[s][i - 1]);
+          else a = cmulc(a, tw(i * j));  // w^(i k), k = j, span QL = H / RL
+        }
+        v[s][i] = a;
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < PER; s++) {
+      const int u = tid + s * T, b = u / QL, j = u % QL;
+      const uint32_t c = cols[b];
+      cplx *col = b ? out1 : out0;
+      dft<RL, true>(v[s]);
+      const cplx tj = twiddle_m(A.twM, c * (uint32_t)j);
+#pragma unroll
+      for (int i = 0; i < RL; i++) {
+        const int row = j + i * QL;
+        const cplx w = i ? cmul(tj, twiddle_m(A.twM, c * (uint32_t)(i * QL))) : tj;
+        gstore(col + (((size_t)(row >> rs) * W << rs) | (row & rmask)), cmulc(v[s][i], w));
+      }
+    }
+  } else {
+    if constexpr (CACHE) fft_lds_cached<H, 2, T, true>(lds, ptw);
+    else fft_lds<H, 2, T, true>(lds, tw);
     const cplx t0 = twiddle_m(A.twM, c0 * (uint32_t)tid), t1 = twiddle_m(A.twM, c1 * (uint32_t)tid);
 #pragma unroll 2
     for (int st = 0; st < H / T; st++) {
