@@ -335,6 +335,14 @@ __device__ __forceinline__ void fft_lds(cplx *buf, const TW &tw) {
   }
 }
 
+// Workgroup barrier for LDS only: waits for this wave's LDS operations, not
+// for its outstanding global loads, which stay in flight across it.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // ------------------------------------------------------ wave-local passes
 // When L = RAD^k, every pass has the same Q = L/RAD butterflies per line and
 // butterfly u = tid + s T works in line u / Q.  If Q divides 64 (or the

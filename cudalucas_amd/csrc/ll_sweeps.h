@@ -180,7 +180,6 @@ __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict_
   const uint32_t rs = A.rshift, rmask = (1u << rs) - 1;
   const cplx *col0 = A.x + ((size_t)c0 << rs);
   const cplx *col1 = A.x + ((size_t)c1 << rs);
-  load_quarter<H, T>(qc, A.qcH);
   const QuarterRoots<H> tw{qc};
   // The four-step twiddle w_M^(c b) between the line and the column DFTs
   // is applied on the way in and undone on the way out.
@@ -193,9 +192,13 @@ __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict_
   // two LDS round trips and their barriers.
   constexpr int R0 = plan_radix(H, 0), Q0 = H / R0;
   constexpr int NP = plan_passes(H), RL = plan_radix(H, NP - 1), QL = H / RL;
-  // (4096-point columns with 512 threads run out of registers when fused)
-  constexpr bool FUSED = (2 * Q0) % T == 0 && (2 * QL) % T == 0 && NP >= 2 && H < 4096;
+  // (measured: no gain for 4096-point columns, whose 512 threads run short
+  // of registers when fused)
+  constexpr bool FUSED = (2 * Q0) % T == 0 && (2 * QL) % T == 0 && NP >= 2 && 2 * (R0 + RL) <= T && H < 4096;
   const uint32_t cols[2] = {c0, c1};
+  // w^(c i Q0) (i < R0) and w^(c i QL) (i < RL) of both columns, shared by
+  // every thread of the column
+  __shared__ cplx colf0[FUSED ? 2 * R0 : 1], colfL[FUSED ? 2 * RL : 1];
   if constexpr (FUSED) {
     constexpr int PER = 2 * Q0 / T;
     cplx v[PER][R0];
@@ -209,16 +212,20 @@ __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict_
         v[s][i] = gload(col + (((size_t)(row >> rs) * W << rs) | (row & rmask)));
       }
     }
+    if (tid < 2 * R0) {
+      colf0[tid] = twiddle_m(A.twM, cols[tid / R0] * (uint32_t)((tid % R0) * Q0));
+    } else if (tid < 2 * (R0 + RL)) {
+      const int t = tid - 2 * R0;
+      colfL[t] = twiddle_m(A.twM, cols[t / RL] * (uint32_t)((t % RL) * QL));
+    }
+    load_quarter<H, T>(qc, A.qcH);
+    lds_barrier();  // the column loads stay in flight
 #pragma unroll
     for (int s = 0; s < PER; s++) {
       const int u = tid + s * T, b = u / Q0, j = u % Q0;
-      const uint32_t c = cols[b];
-      const cplx tj = twiddle_m(A.twM, c * (uint32_t)j);
+      const cplx tj = twiddle_m(A.twM, cols[b] * (uint32_t)j);
 #pragma unroll
-      for (int i = 0; i < R0; i++) {
-        const cplx w = i ? cmul(tj, twiddle_m(A.twM, c * (uint32_t)(i * Q0))) : tj;
-        v[s][i] = cmul(v[s][i], w);
-      }
+      for (int i = 0; i < R0; i++) v[s][i] = cmul(v[s][i], i ? cmul(tj, colf0[b * R0 + i]) : tj);
       dft<R0, false>(v[s]);
 #pragma unroll
       for (int i = 0; i < R0; i++) lds[b * LP + lpad(j * R0 + i)] = v[s][i];
@@ -236,6 +243,7 @@ __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict_
       v0[st] = gload(col0 + off);
       v1[st] = gload(col1 + off);
     }
+    load_quarter<H, T>(qc, A.qcH);
     const cplx t0 = twiddle_m(A.twM, c0 * (uint32_t)tid), t1 = twiddle_m(A.twM, c1 * (uint32_t)tid);
 #pragma unroll
     for (int st = 0; st < NE; st++) {
@@ -311,14 +319,13 @@ __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict_
 #pragma unroll
     for (int s = 0; s < PER; s++) {
       const int u = tid + s * T, b = u / QL, j = u % QL;
-      const uint32_t c = cols[b];
       cplx *col = b ? out1 : out0;
       dft<RL, true>(v[s]);
-      const cplx tj = twiddle_m(A.twM, c * (uint32_t)j);
+      const cplx tj = twiddle_m(A.twM, cols[b] * (uint32_t)j);
 #pragma unroll
       for (int i = 0; i < RL; i++) {
         const int row = j + i * QL;
-        const cplx w = i ? cmul(tj, twiddle_m(A.twM, c * (uint32_t)(i * QL))) : tj;
+        const cplx w = i ? cmul(tj, colfL[b * RL + i]) : tj;
         gstore(col + (((size_t)(row >> rs) * W << rs) | (row & rmask)), cmulc(v[s][i], w));
       }
     }
@@ -365,19 +372,12 @@ __global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArg
   const double dmod = (double)A.st->m2_value;
   const uint32_t blk = blockIdx.x;
 
-  load_quarter<W, T>(qc, A.qcW);
-  __syncthreads();
-  if (tid < 2 * R) {
-    const uint32_t i = 2 * blk * R + tid;
-    fb[tid] = gload(A.fB + i);
-    wb[tid] = gload(A.wB + i);
-    iwb[tid] = gload(A.iwB + i) * A.scale;
-  }
   const uint32_t b0 = blk * R;
   const QuarterRoots<W> tw{qc};
 
-  // load R lines (the square sweep has already undone the column twiddle),
-  // every load of the thread in flight before the first LDS write
+  // load R lines (the square sweep has already undone the column twiddle):
+  // every load of the thread is issued first, then the small tables, so
+  // their latencies overlap
   {
     constexpr int NE = (R * W + T - 1) / T;
     cplx v[NE];
@@ -385,6 +385,13 @@ __global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArg
     for (int s = 0; s < NE; s++) {
       const int e = tid + s * T;
       if ((R * W) % T == 0 || e < R * W) v[s] = gload(A.x + (size_t)b0 * W + e);
+    }
+    load_quarter<W, T>(qc, A.qcW);
+    if (tid < 2 * R) {
+      const uint32_t i = 2 * blk * R + tid;
+      fb[tid] = gload(A.fB + i);
+      wb[tid] = gload(A.wB + i);
+      iwb[tid] = gload(A.iwB + i) * A.scale;
     }
 #pragma unroll
     for (int s = 0; s < NE; s++) {

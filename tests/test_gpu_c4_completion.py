@@ -8,12 +8,13 @@ Mersenne-prime exponent ends at residue 0 and every other one does not;
 no iteration of any test exceeds the 0.35 roundoff limit (so there is no
 undetected failure to look for); and for the smallest exponents the full
 final residue equals the plain single-exponent engine's, run to completion
-beside it.  The run takes about 25 GPU-minutes, so it is split in three
+beside it.  The run takes about 40 GPU-minutes, so it is split in four
 parts and only runs when asked:
 
     LL_C4_COMPLETION=low  python -m pytest -s tests/test_gpu_c4_completion.py   # p < 2e6
     LL_C4_COMPLETION=mid  python -m pytest -s tests/test_gpu_c4_completion.py   # 2e6 <= p < 3.5e6
-    LL_C4_COMPLETION=high python -m pytest -s tests/test_gpu_c4_completion.py   # p >= 3.5e6
+    LL_C4_COMPLETION=high python -m pytest -s tests/test_gpu_c4_completion.py   # 3.5e6 <= p < 4.3e6
+    LL_C4_COMPLETION=top  python -m pytest -s tests/test_gpu_c4_completion.py   # p >= 4.3e6
 """
 import hashlib
 import json
@@ -31,7 +32,8 @@ from conftest import announce
 pytestmark = pytest.mark.gpu
 
 MERSENNE_PRIME_EXPONENTS = [110503, 132049, 216091, 756839, 859433, 1257787, 1398269, 2976221, 3021377]
-PARTS = {"low": (0, 2_000_000), "mid": (2_000_000, 3_500_000), "high": (3_500_000, 10_000_000)}
+PARTS = {"low": (0, 2_000_000), "mid": (2_000_000, 3_500_000), "high": (3_500_000, 4_300_000),
+         "top": (4_300_000, 10_000_000)}
 PLAIN_CHECKS = 24  # smallest exponents also run to completion in the plain engine
 PART = os.environ.get("LL_C4_COMPLETION", "")
 
@@ -40,7 +42,7 @@ def digest(words):
     return hashlib.sha256(np.ascontiguousarray(words, dtype="<u4").tobytes()).hexdigest()[:32]
 
 
-@pytest.mark.skipif(PART not in PARTS, reason="set LL_C4_COMPLETION=low|mid|high (6-10 GPU-minutes each)")
+@pytest.mark.skipif(PART not in PARTS, reason="set LL_C4_COMPLETION=low|mid|high|top (8-12 GPU-minutes each)")
 def test_c4_batch_to_completion():
     ps = W.log_uniform_primes(W.C4_SEED, W.C4_COUNT, *W.C4_RANGE)
     ps = sorted(set(ps) | set(MERSENNE_PRIME_EXPONENTS))
