@@ -344,46 +344,70 @@ __device__ __forceinline__ void lds_barrier() {
 }
 
 // ------------------------------------------------------ wave-local passes
-// When L = RAD^k, every pass has the same Q = L/RAD butterflies per line and
-// butterfly u = tid + s T works in line u / Q.  If Q divides 64 (or the
-// workgroup is one wave) all butterflies of a line belong to one wave in
-// every pass, so the passes need no workgroup barrier: LDS instructions of
-// one wave execute in order, and a wavefront fence keeps the compiler from
-// moving them across a pass boundary.
+// Each wave of the workgroup owns LW = NB / (T / 64) consecutive lines and
+// runs every pass of their transforms by itself: in a pass with Q
+// butterflies per line, lane l handles butterflies v = l + 64 s of its
+// lines (line v / Q, index v mod Q).  When LW Q is a multiple of 64 in
+// every pass, no workgroup barrier is needed between passes: LDS
+// instructions of one wave execute in order, and a wavefront fence keeps
+// the compiler from moving them across a pass boundary.
+// Plans for the wave-local form: UNI = true takes a single radix 8 or 16
+// for every pass when L is a power of it (measured faster for L = 512 than
+// the mixed default plan), UNI = false the default plan.
 constexpr int uniform_radix(int L) {
-  int r = 16;
-  for (; r >= 8; r /= 2) {
+  for (int r = 16; r >= 8; r /= 2) {
     int x = L;
     while (x % r == 0) x /= r;
     if (x == 1) return r;
   }
   return 0;
 }
+template <bool UNI>
+constexpr int wave_radix(int L, int pass) {
+  return UNI ? uniform_radix(L) : plan_radix(L, pass);
+}
+
+template <int L, int NB, int T, bool UNI, int PASS = 0, int NS = 1>
+constexpr bool wave_passes_ok() {
+  if constexpr (NS >= L) {
+    return true;
+  } else {
+    constexpr int R = wave_radix<UNI>(L, PASS);
+    constexpr int LW = NB / (T / 64);
+    if constexpr (R == 0) return false;
+    else return (LW * (L / R)) % 64 == 0 && wave_passes_ok<L, NB, T, UNI, PASS + 1, NS * R>();
+  }
+}
+
+template <int L, int NB, int T, bool UNI>
+constexpr bool wave_plan_ok() {
+  return T % 64 == 0 && NB % (T / 64) == 0 && (!UNI || uniform_radix(L) != 0) &&
+         wave_passes_ok<L, NB, T, UNI>();
+}
 
 template <int L, int NB, int T>
 constexpr bool wave_local_ok() {
-  constexpr int RAD = uniform_radix(L);
-  constexpr int Q = RAD ? L / RAD : 0;
-  return RAD != 0 && ((Q <= 64 && 64 % Q == 0) || T == 64) && T % 64 == 0 && (NB * Q) % T == 0;
+  return wave_plan_ok<L, NB, T, true>() || wave_plan_ok<L, NB, T, false>();
 }
 
-template <int L, int RAD, int NS, int NB, int T, bool INV, typename TW>
+template <int L, int R, int NS, int NB, int T, bool INV, typename TW>
 __device__ __forceinline__ void stockham_pass_wave(cplx *buf, const TW &tw) {
-  constexpr int Q = L / RAD;
-  constexpr int PER = NB * Q / T;
+  constexpr int Q = L / R;
+  constexpr int LW = NB / (T / 64);
+  constexpr int PER = LW * Q / 64;
   constexpr int LP = lpitch(L);
-  cplx v[PER][RAD];
-  const int tid = threadIdx.x;
+  cplx v[PER][R];
+  const int lane = threadIdx.x & 63, line0 = (threadIdx.x >> 6) * LW;
 #pragma unroll
   for (int s = 0; s < PER; s++) {
-    const int u = tid + s * T;
-    const int b = u / Q, j = u % Q;
+    const int u = lane + 64 * s;
+    const int b = line0 + u / Q, j = u % Q;
     const int k = j % NS;
 #pragma unroll
-    for (int i = 0; i < RAD; i++) {
+    for (int i = 0; i < R; i++) {
       cplx a = buf[b * LP + lpad(j + i * Q)];
       if (NS > 1 && i > 0) {
-        const cplx w = tw((i * k) * (L / (NS * RAD)));
+        const cplx w = tw((i * k) * (L / (NS * R)));
         a = INV ? cmulc(a, w) : cmul(a, w);
       }
       v[s][i] = a;
@@ -394,27 +418,28 @@ __device__ __forceinline__ void stockham_pass_wave(cplx *buf, const TW &tw) {
   __builtin_amdgcn_wave_barrier();
 #pragma unroll
   for (int s = 0; s < PER; s++) {
-    const int u = tid + s * T;
-    const int b = u / Q, j = u % Q;
+    const int u = lane + 64 * s;
+    const int b = line0 + u / Q, j = u % Q;
     const int k = j % NS;
-    dft<RAD, INV>(v[s]);
-    const int base = (j / NS) * NS * RAD + k;
+    dft<R, INV>(v[s]);
+    const int base = (j / NS) * NS * R + k;
 #pragma unroll
-    for (int i = 0; i < RAD; i++) buf[b * LP + lpad(base + i * NS)] = v[s][i];
+    for (int i = 0; i < R; i++) buf[b * LP + lpad(base + i * NS)] = v[s][i];
   }
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
   __builtin_amdgcn_wave_barrier();
 }
 
-// NB transforms of length L = RAD^k, each wave transforming its own lines;
-// callers synchronise the workgroup before and after.
-template <int L, int NB, int T, bool INV, int NS = 1, typename TW>
+// NB transforms of length L, each wave transforming its own lines; callers
+// synchronise the workgroup before and after.
+template <int L, int NB, int T, bool INV, bool UNI = wave_plan_ok<L, NB, T, true>(), int PASS = 0,
+          int NS = 1, typename TW>
 __device__ __forceinline__ void fft_lds_wave(cplx *buf, const TW &tw) {
-  static_assert(wave_local_ok<L, NB, T>(), "fft_lds_wave: plan is not wave-local");
+  static_assert(wave_plan_ok<L, NB, T, UNI>(), "fft_lds_wave: plan is not wave-local");
   if constexpr (NS < L) {
-    constexpr int RAD = uniform_radix(L);
-    stockham_pass_wave<L, RAD, NS, NB, T, INV>(buf, tw);
-    fft_lds_wave<L, NB, T, INV, NS * RAD>(buf, tw);
+    constexpr int R = wave_radix<UNI>(L, PASS);
+    stockham_pass_wave<L, R, NS, NB, T, INV>(buf, tw);
+    fft_lds_wave<L, NB, T, INV, UNI, PASS + 1, NS * R>(buf, tw);
   }
 }
 
