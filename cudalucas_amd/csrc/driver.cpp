@@ -661,6 +661,17 @@ static std::string device_file(int device, const char *suffix) {
   return std::string(d.name) + " " + suffix;
 }
 
+// A data file shipped in data/ beside the binary.
+static std::string packaged_file(const char *name) {
+  char exe[4096];
+  const ssize_t k = readlink("/proc/self/exe", exe, sizeof exe - 1);
+  if (k <= 0) return "";
+  exe[k] = 0;
+  std::string dir(exe);
+  const size_t slash = dir.rfind('/');
+  return slash == std::string::npos ? "" : dir.substr(0, slash) + "/data/" + name;
+}
+
 // init_device with show_prop (CUDALucas.cu:737-810)
 static void print_device_info(int device) {
   ll_device_info d;
@@ -1002,7 +1013,18 @@ int main(int argc, char **argv) {
   const int dev = cfg.device < 0 ? 0 : cfg.device;
   if (cfg.info) print_device_info(dev);
   if (cfg.cufftbench[2] > 0) return cufftbench(cfg, cfg.cufftbench[0], cfg.cufftbench[1], cfg.cufftbench[2]);
-  load_schedule_file(device_file(dev, "schedules.txt"));
+  // "<device> schedules.txt" from -threadbench; without one, the table
+  // tuned for this GPU that ships beside the binary
+  {
+    const std::string own = device_file(dev, "schedules.txt");
+    FILE *f = own.empty() ? nullptr : fopen(own.c_str(), "r");
+    if (f) {
+      fclose(f);
+      load_schedule_file(own);
+    } else {
+      load_schedule_file(packaged_file("mi355x_schedules_single.txt"));
+    }
+  }
   if (cfg.threadbench[2] > 0)
     return threadbench(cfg, cfg.threadbench[0], cfg.threadbench[1], cfg.threadbench[2], cfg.threadbench[3]);
   load_fft_file(device_file(dev, "fft.txt"));

@@ -234,8 +234,9 @@ constexpr int log2_pow2(int L) { int k = 0; while ((1 << k) < L) k++; return k; 
 
 // Radix of pass `pass` for a length-L transform: odd prime factors first
 // (7, 5, 3), then the power-of-two part split into 16s with an 8, 4 or
-// 8 x 8 remainder chosen to avoid radix 2.
-constexpr int plan_radix(int L, int pass) {
+// 8 x 8 remainder chosen to avoid radix 2 (maxr = 8: into 8s, with a 4 or
+// 4 x 4 remainder).
+constexpr int plan_radix(int L, int pass, int maxr = 16) {
   int rads[32] = {0};
   int n = 0;
   int o = odd_part(L);
@@ -243,6 +244,14 @@ constexpr int plan_radix(int L, int pass) {
   while (o % 5 == 0) { rads[n++] = 5; o /= 5; }
   while (o % 3 == 0) { rads[n++] = 3; o /= 3; }
   int k = log2_pow2(L / odd_part(L));
+  if (maxr == 8) {
+    int a8 = k / 3, r8 = k % 3;
+    if (r8 == 1 && a8 >= 1) { a8 -= 1; rads[n++] = 4; rads[n++] = 4; }
+    else if (r8 == 2) { rads[n++] = 4; }
+    else if (r8 == 1) { rads[n++] = 2; }
+    for (int i = 0; i < a8; i++) rads[n++] = 8;
+    return pass < n ? rads[pass] : 0;
+  }
   // k = 4a + r
   int r = k % 4;
   int a = k / 4;
@@ -461,24 +470,24 @@ __device__ __forceinline__ void fft_lds_auto(cplx *buf, const TW &tw) {
 // are fixed: they are computed once per kernel and shared by the forward
 // and the inverse transform (the inverse uses their conjugates), which
 // takes the table lookups out of the passes.
-template <int L, int NB, int T, int PASS = 0, int NS = 1, bool END = (NS >= L)>
+template <int L, int NB, int T, int PASS = 0, int NS = 1, int MAXR = 16, bool END = (NS >= L)>
 struct PassTwiddles;
 
-template <int L, int NB, int T, int PASS, int NS>
-struct PassTwiddles<L, NB, T, PASS, NS, true> {
+template <int L, int NB, int T, int PASS, int NS, int MAXR>
+struct PassTwiddles<L, NB, T, PASS, NS, MAXR, true> {
   template <typename TW>
   __device__ __forceinline__ void init(const TW &) {}
 };
 
-template <int L, int NB, int T, int PASS, int NS>
-struct PassTwiddles<L, NB, T, PASS, NS, false> {
-  static constexpr int R = plan_radix(L, PASS);
+template <int L, int NB, int T, int PASS, int NS, int MAXR>
+struct PassTwiddles<L, NB, T, PASS, NS, MAXR, false> {
+  static constexpr int R = plan_radix(L, PASS, MAXR);
   static constexpr int Q = L / R;
   static constexpr int NBF = NB * Q;
   static constexpr int PER = (NBF + T - 1) / T;
   static constexpr int NW = NS > 1 ? R - 1 : 1;
   cplx w[PER][NW];
-  PassTwiddles<L, NB, T, PASS + 1, NS * R> next;
+  PassTwiddles<L, NB, T, PASS + 1, NS * R, MAXR> next;
 
   template <typename TW>
   __device__ __forceinline__ void init(const TW &tw) {
@@ -535,24 +544,24 @@ __device__ __forceinline__ void stockham_pass_cached(cplx *buf, const PT &pt) {
 
 // fft_lds with cached twiddles (pt: the PassTwiddles of the same L, NB, T).
 // Passes PASS .. END-1 are run (END = -1: to the last pass).
-template <int L, int NB, int T, bool INV, int PASS = 0, int NS = 1, int END = -1, typename PT>
+template <int L, int NB, int T, bool INV, int PASS = 0, int NS = 1, int END = -1, int MAXR = 16, typename PT>
 __device__ __forceinline__ void fft_lds_cached(cplx *buf, const PT &pt) {
   if constexpr (NS < L && PASS != END) {
-    constexpr int R = plan_radix(L, PASS);
+    constexpr int R = plan_radix(L, PASS, MAXR);
     stockham_pass_cached<L, R, NS, NB, T, INV>(buf, pt);
-    fft_lds_cached<L, NB, T, INV, PASS + 1, NS * R, END>(buf, pt.next);
+    fft_lds_cached<L, NB, T, INV, PASS + 1, NS * R, END, MAXR>(buf, pt.next);
   }
 }
 
 // Number of passes of the plan of L, and the span NS before pass P.
-constexpr int plan_passes(int L) {
+constexpr int plan_passes(int L, int maxr = 16) {
   int n = 0, ns = 1;
-  while (ns < L) ns *= plan_radix(L, n++);
+  while (ns < L) ns *= plan_radix(L, n++, maxr);
   return n;
 }
-constexpr int plan_span(int L, int P) {
+constexpr int plan_span(int L, int P, int maxr = 16) {
   int ns = 1;
-  for (int i = 0; i < P; i++) ns *= plan_radix(L, i);
+  for (int i = 0; i < P; i++) ns *= plan_radix(L, i, maxr);
   return ns;
 }
 

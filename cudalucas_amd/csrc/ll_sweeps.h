@@ -149,7 +149,7 @@ __device__ __forceinline__ void pair_square(cplx &A, cplx &B, cplx t) {
 // workgroup 0 owns the two self-paired columns 0 and W/2.  After the forward
 // H-point DFT, element k2 of column k1 holds Z[k1 + W k2]; its partner
 // M - k is (W - k1, H - 1 - k2) for k1 != 0 and (0, H - k2) for k1 = 0.
-template <int H, int T>
+template <int H, int T, int MAXR = 16>
 __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict__ args) {
   const PlanArgs &A = args[blockIdx.y];
   const bool finished = A.remaining && *A.remaining == 0;  // a batched test that is done
@@ -190,8 +190,8 @@ __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict_
   // w^(c j) * w^(c Q0 i) -- and runs that pass in registers; the last pass
   // of the inverse transform likewise writes straight to HBM.  That saves
   // two LDS round trips and their barriers.
-  constexpr int R0 = plan_radix(H, 0), Q0 = H / R0;
-  constexpr int NP = plan_passes(H), RL = plan_radix(H, NP - 1), QL = H / RL;
+  constexpr int R0 = plan_radix(H, 0, MAXR), Q0 = H / R0;
+  constexpr int NP = plan_passes(H, MAXR), RL = plan_radix(H, NP - 1, MAXR), QL = H / RL;
   // (measured: no gain for 4096-point columns, whose 512 threads run short
   // of registers when fused)
   constexpr bool FUSED = (2 * Q0) % T == 0 && (2 * QL) % T == 0 && NP >= 2 && 2 * (R0 + RL) <= T && H < 4096;
@@ -258,12 +258,13 @@ __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict_
   // Per-thread twiddles are cached in registers for both transforms (the
   // LDS-table form is kept for shapes where registers run short).
   constexpr bool CACHE = true;
-  PassTwiddles<(CACHE ? H : 16), 2, (CACHE ? T : 64)> ptw;
+  PassTwiddles<(CACHE ? H : 16), 2, (CACHE ? T : 64), 0, 1, MAXR> ptw;
   if constexpr (CACHE) {
     ptw.init(tw);
-    if constexpr (FUSED) fft_lds_cached<H, 2, T, false, 1, R0>(lds, ptw.next);
-    else fft_lds_cached<H, 2, T, false>(lds, ptw);
+    if constexpr (FUSED) fft_lds_cached<H, 2, T, false, 1, R0, -1, MAXR>(lds, ptw.next);
+    else fft_lds_cached<H, 2, T, false, 0, 1, -1, MAXR>(lds, ptw);
   } else {
+    static_assert(MAXR == 16, "table-twiddle form uses the default plan");
     if constexpr (FUSED) fft_lds<H, 2, T, false, 1, R0>(lds, tw);
     else fft_lds<H, 2, T, false>(lds, tw);
   }
@@ -296,10 +297,10 @@ __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict_
   cplx *out0 = A.x + ((size_t)c0 << rs);
   cplx *out1 = A.x + ((size_t)c1 << rs);
   if constexpr (FUSED) {
-    if constexpr (CACHE) fft_lds_cached<H, 2, T, true, 0, 1, NP - 1>(lds, ptw);
+    if constexpr (CACHE) fft_lds_cached<H, 2, T, true, 0, 1, NP - 1, MAXR>(lds, ptw);
     else fft_lds<H, 2, T, true, 0, 1, NP - 1>(lds, tw);
     // last inverse pass: rows j + i QL of each column, straight to HBM
-    constexpr int NSL = plan_span(H, NP - 1);
+    constexpr int NSL = plan_span(H, NP - 1, MAXR);
     static_assert(NSL == QL, "last pass spans the whole column");
     constexpr int PER = 2 * QL / T;
     cplx v[PER][RL];
@@ -330,7 +331,7 @@ __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict_
       }
     }
   } else {
-    if constexpr (CACHE) fft_lds_cached<H, 2, T, true>(lds, ptw);
+    if constexpr (CACHE) fft_lds_cached<H, 2, T, true, 0, 1, -1, MAXR>(lds, ptw);
     else fft_lds<H, 2, T, true>(lds, tw);
     const cplx t0 = twiddle_m(A.twM, c0 * (uint32_t)tid), t1 = twiddle_m(A.twM, c1 * (uint32_t)tid);
 #pragma unroll 2
