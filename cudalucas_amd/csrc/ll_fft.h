@@ -7,6 +7,8 @@
 // roots exp(-2 pi i e/L); inverse passes use their conjugates.
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 #include <stdint.h>
 
 #include "ll_types.h"
@@ -234,9 +236,8 @@ constexpr int log2_pow2(int L) { int k = 0; while ((1 << k) < L) k++; return k; 
 
 // Radix of pass `pass` for a length-L transform: odd prime factors first
 // (7, 5, 3), then the power-of-two part split into 16s with an 8, 4 or
-// 8 x 8 remainder chosen to avoid radix 2 (maxr = 8: into 8s, with a 4 or
-// 4 x 4 remainder).
-constexpr int plan_radix(int L, int pass, int maxr = 16) {
+// 8 x 8 remainder chosen to avoid radix 2.
+constexpr int plan_radix(int L, int pass) {
   int rads[32] = {0};
   int n = 0;
   int o = odd_part(L);
@@ -244,14 +245,6 @@ constexpr int plan_radix(int L, int pass, int maxr = 16) {
   while (o % 5 == 0) { rads[n++] = 5; o /= 5; }
   while (o % 3 == 0) { rads[n++] = 3; o /= 3; }
   int k = log2_pow2(L / odd_part(L));
-  if (maxr == 8) {
-    int a8 = k / 3, r8 = k % 3;
-    if (r8 == 1 && a8 >= 1) { a8 -= 1; rads[n++] = 4; rads[n++] = 4; }
-    else if (r8 == 2) { rads[n++] = 4; }
-    else if (r8 == 1) { rads[n++] = 2; }
-    for (int i = 0; i < a8; i++) rads[n++] = 8;
-    return pass < n ? rads[pass] : 0;
-  }
   // k = 4a + r
   int r = k % 4;
   int a = k / 4;
@@ -399,6 +392,29 @@ constexpr bool wave_local_ok() {
   return wave_plan_ok<L, NB, T, true>() || wave_plan_ok<L, NB, T, false>();
 }
 
+// Twiddles of every pass of one plan, laid out pass by pass: the pass of
+// span NS and radix R keeps w^(i k), i in [1, R), k in [0, NS), at
+// pt[NS - 1 + (i - 1) NS + k] (the earlier passes fill the first NS - 1
+// entries, so a length-L plan needs L - 1).  Lanes of a pass read
+// consecutive k, so the lookups are contiguous LDS reads.
+struct WaveTable {
+  const cplx *pt;
+};
+
+template <int L, int T, bool UNI, int PASS = 0, int NS = 1, typename TW>
+__device__ __forceinline__ void build_wave_table(cplx *pt, const TW &tw) {
+  if constexpr (NS < L) {
+    constexpr int R = wave_radix<UNI>(L, PASS);
+    if constexpr (NS > 1) {
+      for (int idx = threadIdx.x; idx < (R - 1) * NS; idx += T) {
+        const int i = idx / NS + 1, k = idx % NS;
+        pt[NS - 1 + idx] = tw((i * k) * (L / (NS * R)));
+      }
+    }
+    build_wave_table<L, T, UNI, PASS + 1, NS * R>(pt, tw);
+  }
+}
+
 template <int L, int R, int NS, int NB, int T, bool INV, typename TW>
 __device__ __forceinline__ void stockham_pass_wave(cplx *buf, const TW &tw) {
   constexpr int Q = L / R;
@@ -416,7 +432,9 @@ __device__ __forceinline__ void stockham_pass_wave(cplx *buf, const TW &tw) {
     for (int i = 0; i < R; i++) {
       cplx a = buf[b * LP + lpad(j + i * Q)];
       if (NS > 1 && i > 0) {
-        const cplx w = tw((i * k) * (L / (NS * R)));
+        cplx w;
+        if constexpr (std::is_same<TW, WaveTable>::value) w = tw.pt[NS - 1 + (i - 1) * NS + k];
+        else w = tw((i * k) * (L / (NS * R)));
         a = INV ? cmulc(a, w) : cmul(a, w);
       }
       v[s][i] = a;
@@ -452,16 +470,26 @@ __device__ __forceinline__ void fft_lds_wave(cplx *buf, const TW &tw) {
   }
 }
 
-// Either form, chosen at compile time.
-template <int L, int NB, int T, bool INV, typename TW>
-__device__ __forceinline__ void fft_lds_auto(cplx *buf, const TW &tw) {
+// Either form, chosen at compile time.  With TAB the wave-local form takes
+// its twiddles from the per-pass table `pt` (build_auto_table, L - 1
+// entries, built before the first transform), else from `tw`.
+template <int L, int NB, int T, bool INV, bool TAB, typename TW>
+__device__ __forceinline__ void fft_lds_auto(cplx *buf, const TW &tw, const cplx *pt) {
   if constexpr (wave_local_ok<L, NB, T>()) {
     __syncthreads();
-    fft_lds_wave<L, NB, T, INV>(buf, tw);
+    if constexpr (TAB) fft_lds_wave<L, NB, T, INV>(buf, WaveTable{pt});
+    else fft_lds_wave<L, NB, T, INV>(buf, tw);
     __syncthreads();
   } else {
     fft_lds<L, NB, T, INV>(buf, tw);
   }
+}
+
+// Builds the table fft_lds_auto<..., TAB = true> reads.
+template <int L, int NB, int T, bool TAB, typename TW>
+__device__ __forceinline__ void build_auto_table(cplx *pt, const TW &tw) {
+  if constexpr (TAB && wave_local_ok<L, NB, T>())
+    build_wave_table<L, T, wave_plan_ok<L, NB, T, true>()>(pt, tw);
 }
 
 // ------------------------------------------------ per-thread twiddle cache
@@ -470,24 +498,24 @@ __device__ __forceinline__ void fft_lds_auto(cplx *buf, const TW &tw) {
 // are fixed: they are computed once per kernel and shared by the forward
 // and the inverse transform (the inverse uses their conjugates), which
 // takes the table lookups out of the passes.
-template <int L, int NB, int T, int PASS = 0, int NS = 1, int MAXR = 16, bool END = (NS >= L)>
+template <int L, int NB, int T, int PASS = 0, int NS = 1, bool END = (NS >= L)>
 struct PassTwiddles;
 
-template <int L, int NB, int T, int PASS, int NS, int MAXR>
-struct PassTwiddles<L, NB, T, PASS, NS, MAXR, true> {
+template <int L, int NB, int T, int PASS, int NS>
+struct PassTwiddles<L, NB, T, PASS, NS, true> {
   template <typename TW>
   __device__ __forceinline__ void init(const TW &) {}
 };
 
-template <int L, int NB, int T, int PASS, int NS, int MAXR>
-struct PassTwiddles<L, NB, T, PASS, NS, MAXR, false> {
-  static constexpr int R = plan_radix(L, PASS, MAXR);
+template <int L, int NB, int T, int PASS, int NS>
+struct PassTwiddles<L, NB, T, PASS, NS, false> {
+  static constexpr int R = plan_radix(L, PASS);
   static constexpr int Q = L / R;
   static constexpr int NBF = NB * Q;
   static constexpr int PER = (NBF + T - 1) / T;
   static constexpr int NW = NS > 1 ? R - 1 : 1;
   cplx w[PER][NW];
-  PassTwiddles<L, NB, T, PASS + 1, NS * R, MAXR> next;
+  PassTwiddles<L, NB, T, PASS + 1, NS * R> next;
 
   template <typename TW>
   __device__ __forceinline__ void init(const TW &tw) {
@@ -544,24 +572,24 @@ __device__ __forceinline__ void stockham_pass_cached(cplx *buf, const PT &pt) {
 
 // fft_lds with cached twiddles (pt: the PassTwiddles of the same L, NB, T).
 // Passes PASS .. END-1 are run (END = -1: to the last pass).
-template <int L, int NB, int T, bool INV, int PASS = 0, int NS = 1, int END = -1, int MAXR = 16, typename PT>
+template <int L, int NB, int T, bool INV, int PASS = 0, int NS = 1, int END = -1, typename PT>
 __device__ __forceinline__ void fft_lds_cached(cplx *buf, const PT &pt) {
   if constexpr (NS < L && PASS != END) {
-    constexpr int R = plan_radix(L, PASS, MAXR);
+    constexpr int R = plan_radix(L, PASS);
     stockham_pass_cached<L, R, NS, NB, T, INV>(buf, pt);
-    fft_lds_cached<L, NB, T, INV, PASS + 1, NS * R, END, MAXR>(buf, pt.next);
+    fft_lds_cached<L, NB, T, INV, PASS + 1, NS * R, END>(buf, pt.next);
   }
 }
 
 // Number of passes of the plan of L, and the span NS before pass P.
-constexpr int plan_passes(int L, int maxr = 16) {
+constexpr int plan_passes(int L) {
   int n = 0, ns = 1;
-  while (ns < L) ns *= plan_radix(L, n++, maxr);
+  while (ns < L) ns *= plan_radix(L, n++);
   return n;
 }
-constexpr int plan_span(int L, int P, int maxr = 16) {
+constexpr int plan_span(int L, int P) {
   int ns = 1;
-  for (int i = 0; i < P; i++) ns *= plan_radix(L, i, maxr);
+  for (int i = 0; i < P; i++) ns *= plan_radix(L, i);
   return ns;
 }
 

@@ -149,7 +149,7 @@ __device__ __forceinline__ void pair_square(cplx &A, cplx &B, cplx t) {
 // workgroup 0 owns the two self-paired columns 0 and W/2.  After the forward
 // H-point DFT, element k2 of column k1 holds Z[k1 + W k2]; its partner
 // M - k is (W - k1, H - 1 - k2) for k1 != 0 and (0, H - k2) for k1 = 0.
-template <int H, int T, int MAXR = 16>
+template <int H, int T>
 __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict__ args) {
   const PlanArgs &A = args[blockIdx.y];
   const bool finished = A.remaining && *A.remaining == 0;  // a batched test that is done
@@ -190,8 +190,8 @@ __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict_
   // w^(c j) * w^(c Q0 i) -- and runs that pass in registers; the last pass
   // of the inverse transform likewise writes straight to HBM.  That saves
   // two LDS round trips and their barriers.
-  constexpr int R0 = plan_radix(H, 0, MAXR), Q0 = H / R0;
-  constexpr int NP = plan_passes(H, MAXR), RL = plan_radix(H, NP - 1, MAXR), QL = H / RL;
+  constexpr int R0 = plan_radix(H, 0), Q0 = H / R0;
+  constexpr int NP = plan_passes(H), RL = plan_radix(H, NP - 1), QL = H / RL;
   // (measured: no gain for 4096-point columns, whose 512 threads run short
   // of registers when fused)
   constexpr bool FUSED = (2 * Q0) % T == 0 && (2 * QL) % T == 0 && NP >= 2 && 2 * (R0 + RL) <= T && H < 4096;
@@ -255,19 +255,12 @@ __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict_
     }
   }
   __syncthreads();
-  // Per-thread twiddles are cached in registers for both transforms (the
-  // LDS-table form is kept for shapes where registers run short).
-  constexpr bool CACHE = true;
-  PassTwiddles<(CACHE ? H : 16), 2, (CACHE ? T : 64), 0, 1, MAXR> ptw;
-  if constexpr (CACHE) {
-    ptw.init(tw);
-    if constexpr (FUSED) fft_lds_cached<H, 2, T, false, 1, R0, -1, MAXR>(lds, ptw.next);
-    else fft_lds_cached<H, 2, T, false, 0, 1, -1, MAXR>(lds, ptw);
-  } else {
-    static_assert(MAXR == 16, "table-twiddle form uses the default plan");
-    if constexpr (FUSED) fft_lds<H, 2, T, false, 1, R0>(lds, tw);
-    else fft_lds<H, 2, T, false>(lds, tw);
-  }
+  // Each thread's Stockham twiddles, cached in registers for both transforms
+  // (the inverse uses their conjugates).
+  PassTwiddles<H, 2, T> ptw;
+  ptw.init(tw);
+  if constexpr (FUSED) fft_lds_cached<H, 2, T, false, 1, R0>(lds, ptw.next);
+  else fft_lds_cached<H, 2, T, false>(lds, ptw);
   if (g != 0) {
     for (int k2 = tid; k2 < H; k2 += T) {
       const int k2b = H - 1 - k2;
@@ -297,10 +290,9 @@ __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict_
   cplx *out0 = A.x + ((size_t)c0 << rs);
   cplx *out1 = A.x + ((size_t)c1 << rs);
   if constexpr (FUSED) {
-    if constexpr (CACHE) fft_lds_cached<H, 2, T, true, 0, 1, NP - 1, MAXR>(lds, ptw);
-    else fft_lds<H, 2, T, true, 0, 1, NP - 1>(lds, tw);
+    fft_lds_cached<H, 2, T, true, 0, 1, NP - 1>(lds, ptw);
     // last inverse pass: rows j + i QL of each column, straight to HBM
-    constexpr int NSL = plan_span(H, NP - 1, MAXR);
+    constexpr int NSL = plan_span(H, NP - 1);
     static_assert(NSL == QL, "last pass spans the whole column");
     constexpr int PER = 2 * QL / T;
     cplx v[PER][RL];
@@ -310,10 +302,7 @@ __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict_
 #pragma unroll
       for (int i = 0; i < RL; i++) {
         cplx a = lds[b * LP + lpad(j + i * QL)];
-        if (i > 0) {
-          if constexpr (CACHE) a = cmulc(a, twiddle_level<NP - 1>(ptw).w[s][i - 1]);
-          else a = cmulc(a, tw(i * j));  // w^(i k), k = j, span QL = H / RL
-        }
+        if (i > 0) a = cmulc(a, twiddle_level<NP - 1>(ptw).w[s][i - 1]);
         v[s][i] = a;
       }
     }
@@ -331,8 +320,7 @@ __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict_
       }
     }
   } else {
-    if constexpr (CACHE) fft_lds_cached<H, 2, T, true, 0, 1, -1, MAXR>(lds, ptw);
-    else fft_lds<H, 2, T, true>(lds, tw);
+    fft_lds_cached<H, 2, T, true>(lds, ptw);
     const cplx t0 = twiddle_m(A.twM, c0 * (uint32_t)tid), t1 = twiddle_m(A.twM, c1 * (uint32_t)tid);
 #pragma unroll 2
     for (int st = 0; st < H / T; st++) {
@@ -354,6 +342,23 @@ __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict_
 // 2^p = 1) after publishing its own; no block waits on block 0 finishing.
 // Should a wait ever outlast kSpinLimit polls, the iteration is reported as
 // failed (error 3.0) rather than hanging the device.
+// The carry sweep keeps per-pass twiddle tables (16 W bytes of LDS) for its
+// wave-local FFT passes when they cost no workgroup per CU (160 KiB of LDS
+// and 2048 threads per CU); the LDS bytes below are the kernel's own arrays.
+template <int W, int R, int T>
+constexpr bool carry_pass_table() {
+  if constexpr (!wave_local_ok<W, R, T>()) {
+    return false;
+  } else {
+    constexpr int kLds = 163840, cap = 2048 / T;
+    constexpr int base = R * lpitch(W) * 16 + (W / 4 + 1) * 8 + 2 * R * 4 + 4 * R * 8 + (T / 64) * 4;
+    constexpr int with = base + 16 * W;
+    constexpr int a = kLds / base < cap ? kLds / base : cap;
+    constexpr int b = kLds / with < cap ? kLds / with : cap;
+    return with <= kLds && b >= a;
+  }
+}
+
 template <int W, int R, int T>
 __global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArgs *__restrict__ args) {
   const PlanArgs &A = args[blockIdx.y];
@@ -365,6 +370,8 @@ __global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArg
   __shared__ int fb[2 * R];  // weight factors shared by every column
   __shared__ double wb[2 * R], iwb[2 * R];
   __shared__ float red[T / 64];
+  constexpr bool TAB = carry_pass_table<W, R, T>();
+  __shared__ cplx ptab[TAB ? W : 1];  // wave-local pass twiddles
   const int tid = threadIdx.x;
   const uint32_t H = A.H;
   const uint32_t G = H / R;
@@ -378,7 +385,12 @@ __global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArg
 
   // load R lines (the square sweep has already undone the column twiddle):
   // every load of the thread is issued first, then the small tables, so
-  // their latencies overlap
+  // their latencies overlap.  Workgroups of 512 or more threads (one per
+  // CU, registers to spare) also fetch here the per-column weights the
+  // carry step uses after the inverse transform.
+  constexpr bool PF = T >= 512;
+  int fa_c[PF ? CPT : 1];
+  double iwa_c[PF ? CPT : 1], wa_c[PF ? CPT : 1];
   {
     constexpr int NE = (R * W + T - 1) / T;
     cplx v[NE];
@@ -388,6 +400,15 @@ __global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArg
       if ((R * W) % T == 0 || e < R * W) v[s] = gload(A.x + (size_t)b0 * W + e);
     }
     load_quarter<W, T>(qc, A.qcW);
+#pragma unroll
+    for (int c = 0; c < (PF ? CPT : 0); c++) {
+      const int a = tid + c * T;
+      if (W % T == 0 || a < W) {
+        fa_c[c] = gload(A.fA + a);
+        iwa_c[c] = gload(A.iwA + a);
+        wa_c[c] = gload(A.wA + a);
+      }
+    }
     if (tid < 2 * R) {
       const uint32_t i = 2 * blk * R + tid;
       fb[tid] = gload(A.fB + i);
@@ -401,16 +422,15 @@ __global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArg
     }
   }
   __syncthreads();
-  fft_lds_auto<W, R, T, true>(lds, tw);
+  build_auto_table<W, R, T, TAB>(ptab, tw);
+  fft_lds_auto<W, R, T, true, TAB>(lds, tw, ptab);
 
   // Round, subtract, carry along the 2R consecutive words of each column,
   // in exact FP64 (balance_f64).  The balanced digits go back into the
   // column's own LDS slots until the carry-in arrives.
   const double base_lo = A.base_lo, ibase_lo = 1.0 / base_lo;  // 2^q and 2^-q
   double emax = 0.0;
-  for (int a = tid; a < W; a += T) {
-    const int fa = gload(A.fA + a);
-    const double iwa = gload(A.iwA + a);
+  auto round_column = [&](const int a, const int fa, const double iwa) {
     const uint32_t j0 = 2u * ((uint32_t)a * H + b0);
     double carry = 0.0;
 #pragma unroll
@@ -438,11 +458,18 @@ __global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArg
     const uint64_t word = ((uint64_t)epoch << 32) | (uint32_t)c32;
     __hip_atomic_store(&A.shuttle[(size_t)blk * W + a], word, __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
+  };
+  if constexpr (PF) {
+#pragma unroll
+    for (int c = 0; c < CPT; c++)
+      if (W % T == 0 || tid + c * T < W) round_column(tid + c * T, fa_c[c], iwa_c[c]);
+  } else {
+    for (int a = tid; a < W; a += T) round_column(a, gload(A.fA + a), gload(A.iwA + a));
   }
 
   // carry-in from the previous block (block 0: the last block, one column
   // to the left), propagated through the whole chain, then reweight
-  for (int a = tid; a < W; a += T) {
+  auto carry_in_column = [&](const int a, const int fa, const double wa) {
     const size_t src = blk == 0 ? (size_t)(G - 1) * W + (a + W - 1) % W : (size_t)(blk - 1) * W + a;
     uint64_t word;
     uint32_t spins = 0;
@@ -461,8 +488,6 @@ __global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArg
     // through the whole chain; anything left after 2R words (0 in
     // practice) stays in the last word.
     double cc = (double)(int32_t)(uint32_t)word;
-    const int fa = gload(A.fA + a);
-    const double wa = gload(A.wA + a);
 #pragma unroll
     for (int r = 0; r < R; r++) {
       const cplx dv = lds[r * LP + lpad(a)];
@@ -481,9 +506,16 @@ __global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArg
       }
       lds[r * LP + lpad(a)] = {x[0], x[1]};
     }
+  };
+  if constexpr (PF) {
+#pragma unroll
+    for (int c = 0; c < CPT; c++)
+      if (W % T == 0 || tid + c * T < W) carry_in_column(tid + c * T, fa_c[c], wa_c[c]);
+  } else {
+    for (int a = tid; a < W; a += T) carry_in_column(a, gload(A.fA + a), gload(A.wA + a));
   }
   __syncthreads();
-  fft_lds_auto<W, R, T, false>(lds, tw);
+  fft_lds_auto<W, R, T, false, TAB>(lds, tw, ptab);
   {
     constexpr int NE = (R * W + T - 1) / T;
     cplx v[NE];
