@@ -192,6 +192,11 @@ static int build_sweeps(ll_ctx *c) {
   if ((rc = upload(c, (double **)&A.qcW, qc))) return rc;
   quarter_cosines(H, qc);
   if ((rc = upload(c, (double **)&A.qcH, qc))) return rc;
+  cplx *ptH = nullptr;
+  if ((rc = dalloc(c, &ptH, H))) return rc;
+  HIPCHK(c, launch_square_table(c->sch.sk, A.qcH, ptH, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  A.ptH = ptH;
   // two-level roots of order M
   const uint32_t shift = std::min<uint32_t>(kTwoLevelShift, (uint32_t)log2((double)M));
   const uint32_t S = 1u << shift;

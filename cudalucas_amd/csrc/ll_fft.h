@@ -505,6 +505,7 @@ template <int L, int NB, int T, int PASS, int NS>
 struct PassTwiddles<L, NB, T, PASS, NS, true> {
   template <typename TW>
   __device__ __forceinline__ void init(const TW &) {}
+  __device__ __forceinline__ void init_table(const cplx *) {}
 };
 
 template <int L, int NB, int T, int PASS, int NS>
@@ -529,6 +530,21 @@ struct PassTwiddles<L, NB, T, PASS, NS, false> {
       }
     }
     next.init(tw);
+  }
+
+  // The same twiddles read from the plan's per-pass table (build_wave_table
+  // of the default plan, in global memory).
+  __device__ __forceinline__ void init_table(const cplx *__restrict__ pt) {
+    if constexpr (NS > 1) {
+#pragma unroll
+      for (int s = 0; s < PER; s++) {
+        const int u = (int)threadIdx.x + s * T;
+        const int k = (u % Q) % NS;
+#pragma unroll
+        for (int i = 1; i < R; i++) w[s][i - 1] = gload(pt + (NS - 1 + (i - 1) * NS + k));
+      }
+    }
+    next.init_table(pt);
   }
 };
 

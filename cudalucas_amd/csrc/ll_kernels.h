@@ -32,6 +32,7 @@ struct PlanArgs {
   cplx *x;                 // M complex; element a*H + b at ((b/R) W + a) R + b mod R
   const double *qcW;       // cos(2 pi e/W), e <= W/4
   const double *qcH;       // cos(2 pi e/H), e <= H/4
+  const cplx *ptH;         // column plan's per-pass twiddles, H - 1 roots
   TwoLevel twM;            // exp(-2 pi i e/M), e < M
   const double *wA, *iwA;  // 2^(+-fA[a]/n), a < W
   const int32_t *fA;       // (-b * 2H a) mod n
@@ -65,6 +66,7 @@ struct SquareKernel {
   uint32_t h;
   const void *fn;
   uint32_t threads;
+  const void *table;  // builds the per-pass twiddles of the column plan
 };
 
 struct CarryKernel {
@@ -102,5 +104,7 @@ hipError_t launch_resident_load(const ResidentDesc *d_descs, uint32_t count, con
                                 hipStream_t s);
 hipError_t launch_fill(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
 hipError_t launch_copy(const cplx *in, cplx *out, size_t n, hipStream_t s);
+// ptH of a column plan from its quarter-wave table qcH (both device memory)
+hipError_t launch_square_table(const SquareKernel *k, const double *qcH, cplx *ptH, hipStream_t s);
 
 }  // namespace llamd

@@ -149,10 +149,19 @@ __global__ void fill_u32(uint32_t *p, uint32_t v, size_t n) {
     p[i] = v;
 }
 
+// Per-pass twiddles of the H-point column plan (the square sweep's
+// PassTwiddles::init_table layout), built once per plan from the
+// quarter-wave table, so every twiddle equals the one the quarter table
+// gives.
+template <int H>
+__global__ __launch_bounds__(256) void column_pass_table(const double *qc, cplx *out) {
+  build_wave_table<H, 256, false>(out, QuarterRoots<H>{qc});
+}
+
 // ------------------------------------------------------------- dispatch
 
-#define LL_SQUARE(H) {H, (const void *)square_sweep<H, 256>, 256}
-#define LL_SQUARE_T(H, T) {H, (const void *)square_sweep<H, T>, T}
+#define LL_SQUARE(H) {H, (const void *)square_sweep<H, 256>, 256, (const void *)column_pass_table<H>}
+#define LL_SQUARE_T(H, T) {H, (const void *)square_sweep<H, T>, T, (const void *)column_pass_table<H>}
 #define LL_RESIDENT(M, T) {M, (const void *)resident_kernel<M, T>, T}
 
 static const SquareKernel k_square[] = {
@@ -250,6 +259,11 @@ hipError_t launch_resident_load(const ResidentDesc *d_descs, uint32_t count, con
                                 hipStream_t s) {
   hipLaunchKernelGGL(resident_load, dim3(16, count), dim3(256), 0, s, d_descs, digits);
   return hipGetLastError();
+}
+
+hipError_t launch_square_table(const SquareKernel *k, const double *qcH, cplx *ptH, hipStream_t s) {
+  void *args[] = {&qcH, &ptH};
+  return hipLaunchKernel(k->table, dim3(1), dim3(256), args, 0, s);
 }
 
 hipError_t launch_copy(const cplx *in, cplx *out, size_t n, hipStream_t s) {

@@ -170,7 +170,6 @@ __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict_
   if (finished) return;
   constexpr int LP = lpitch(H);
   __shared__ cplx lds[2 * LP];
-  __shared__ double qc[H / 4 + 1];
   const int tid = threadIdx.x;
   const uint32_t W = A.W;
   const uint32_t g = blockIdx.x;
@@ -180,7 +179,6 @@ __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict_
   const uint32_t rs = A.rshift, rmask = (1u << rs) - 1;
   const cplx *col0 = A.x + ((size_t)c0 << rs);
   const cplx *col1 = A.x + ((size_t)c1 << rs);
-  const QuarterRoots<H> tw{qc};
   // The four-step twiddle w_M^(c b) between the line and the column DFTs
   // is applied on the way in and undone on the way out.
   //
@@ -218,7 +216,6 @@ __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict_
       const int t = tid - 2 * R0;
       colfL[t] = twiddle_m(A.twM, cols[t / RL] * (uint32_t)((t % RL) * QL));
     }
-    load_quarter<H, T>(qc, A.qcH);
     lds_barrier();  // the column loads stay in flight
 #pragma unroll
     for (int s = 0; s < PER; s++) {
@@ -243,7 +240,6 @@ __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict_
       v0[st] = gload(col0 + off);
       v1[st] = gload(col1 + off);
     }
-    load_quarter<H, T>(qc, A.qcH);
     const cplx t0 = twiddle_m(A.twM, c0 * (uint32_t)tid), t1 = twiddle_m(A.twM, c1 * (uint32_t)tid);
 #pragma unroll
     for (int st = 0; st < NE; st++) {
@@ -256,9 +252,9 @@ __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict_
   }
   __syncthreads();
   // Each thread's Stockham twiddles, cached in registers for both transforms
-  // (the inverse uses their conjugates).
+  // (the inverse uses their conjugates), read from the plan's per-pass table.
   PassTwiddles<H, 2, T> ptw;
-  ptw.init(tw);
+  ptw.init_table(A.ptH);
   if constexpr (FUSED) fft_lds_cached<H, 2, T, false, 1, R0>(lds, ptw.next);
   else fft_lds_cached<H, 2, T, false>(lds, ptw);
   if (g != 0) {
