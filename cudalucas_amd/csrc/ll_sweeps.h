@@ -64,9 +64,12 @@ __device__ __forceinline__ int64_t balance(int64_t v, int width, int64_t &carry)
 // integer and the base 2^width a power of two, so v * 2^-width, its rint and
 // the fma below are all exact.  Ties round to even, so the digit lies in
 // [-2^(width-1), 2^(width-1)].
-__device__ __forceinline__ double balance_f64(double v, double base, double inv_base, double &carry) {
-  carry = rint(v * inv_base);
-  return fma(-carry, base, v);
+// Balanced digit of the integer-valued v in base 2^w and the carry out,
+// exactly: v 2^-w and carry 2^w are exact scalings, and v - carry 2^w is
+// an integer of magnitude <= 2^(w-1).
+__device__ __forceinline__ double balance_f64(double v, int w, double &carry) {
+  carry = rint(ldexp(v, -w));
+  return v - ldexp(carry, w);
 }
 
 __device__ __forceinline__ float wave_max(float v) {
@@ -424,30 +427,37 @@ __global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArg
   // Round, subtract, carry along the 2R consecutive words of each column,
   // in exact FP64 (balance_f64).  The balanced digits go back into the
   // column's own LDS slots until the carry-in arrives.
-  const double base_lo = A.base_lo, ibase_lo = 1.0 / base_lo;  // 2^q and 2^-q
+  // Word 2 H a + i of the column has weight exponent f = (fa + fb[i]) mod n
+  // (wrap: the sum reached n, a factor 2 in the weight) and width q + 1
+  // below b, q above; the factors of 2 are exact ldexp scalings.
+  const int q = (int)A.q;
   double emax = 0.0;
   auto round_column = [&](const int a, const int fa, const double iwa) {
-    const uint32_t j0 = 2u * ((uint32_t)a * H + b0);
+    // m2i < 2R: this chain takes the -2, at word m2i (one chain of all)
+    const uint32_t m2i = m2_digit - 2u * ((uint32_t)a * H + b0);
     double carry = 0.0;
+    auto chain = [&](auto with_m2) {
 #pragma unroll
-    for (int r = 0; r < R; r++) {
-      const cplx v = lds[r * LP + lpad(a)];
-      double d[2];
+      for (int r = 0; r < R; r++) {
+        const cplx v = lds[r * LP + lpad(a)];
+        double d[2];
 #pragma unroll
-      for (int e = 0; e < 2; e++) {
-        const int i = 2 * r + e;
-        int f = fa + fb[i];
-        const bool wrap = f >= (int)A.n;
-        f -= wrap ? (int)A.n : 0;
-        const bool big = f < (int)A.b;
-        const double t = (e ? v.y : v.x) * (iwa * iwb[i] * (wrap ? 2.0 : 1.0));
-        double rt = rint(t);
-        emax = fmax(emax, fabs(t - rt));
-        if (j0 + i == m2_digit) rt -= dmod;
-        d[e] = balance_f64(rt + carry, big ? 2.0 * base_lo : base_lo, big ? 0.5 * ibase_lo : ibase_lo, carry);
+        for (int e = 0; e < 2; e++) {
+          const int i = 2 * r + e;
+          const int f0 = fa + fb[i];
+          const int wrap = f0 >= (int)A.n;
+          const int big = f0 - (wrap ? (int)A.n : 0) < (int)A.b;
+          const double t = ldexp((e ? v.y : v.x) * (iwa * iwb[i]), wrap);
+          double rt = rint(t);
+          emax = fmax(emax, fabs(t - rt));
+          if constexpr (decltype(with_m2)::value) rt -= (uint32_t)i == m2i ? dmod : 0.0;
+          d[e] = balance_f64(rt + carry, q + big, carry);
+        }
+        lds[r * LP + lpad(a)] = {d[0], d[1]};
       }
-      lds[r * LP + lpad(a)] = {d[0], d[1]};
-    }
+    };
+    if (m2i < 2u * R) chain(std::true_type{});
+    else chain(std::false_type{});
     const bool fits = fabs(carry) < 2147483648.0;
     if (!fits) emax = 2.0;  // carry overflow: flagged as a failed iteration
     const int32_t c32 = fits ? (int32_t)carry : 0;
@@ -491,14 +501,12 @@ __global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArg
 #pragma unroll
       for (int e = 0; e < 2; e++) {
         const int i = 2 * r + e;
-        int f = fa + fb[i];
-        const bool wrap = f >= (int)A.n;
-        f -= wrap ? (int)A.n : 0;
-        const bool big = f < (int)A.b;
-        const double base = big ? 2.0 * base_lo : base_lo;
-        double d = balance_f64((e ? dv.y : dv.x) + cc, base, big ? 0.5 * ibase_lo : ibase_lo, cc);
-        if (i == 2 * R - 1) d = fma(cc, base, d);
-        x[e] = d * (wa * wb[i] * (wrap ? 0.5 : 1.0));
+        const int f0 = fa + fb[i];
+        const int wrap = f0 >= (int)A.n;
+        const int width = q + (f0 - (wrap ? (int)A.n : 0) < (int)A.b);
+        double d = balance_f64((e ? dv.y : dv.x) + cc, width, cc);
+        if (i == 2 * R - 1) d += ldexp(cc, width);
+        x[e] = ldexp(d * (wa * wb[i]), -wrap);
       }
       lds[r * LP + lpad(a)] = {x[0], x[1]};
     }
