@@ -5,7 +5,8 @@ namespace llamd {
 
 #define LL_CARRY_T(W, R, T)                                                              \
   {W, R, (const void *)carry_sweep<W, R, T>, (const void *)load_lines<W, R, T>,          \
-   (const void *)unload_lines<W, R, T>, T}
+   (const void *)unload_lines<W, R, T>, T,                                               \
+   carry_pass_table<W, R, T>() ? (const void *)line_pass_table<W, R, T> : nullptr}
 #define LL_CARRY(W, R) LL_CARRY_T(W, R, 256)
 
 extern const CarryKernel k_carry_small[] = {

@@ -195,8 +195,15 @@ static int build_sweeps(ll_ctx *c) {
   cplx *ptH = nullptr;
   if ((rc = dalloc(c, &ptH, H))) return rc;
   HIPCHK(c, launch_square_table(c->sch.sk, A.qcH, ptH, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
   A.ptH = ptH;
+  A.ptW = nullptr;
+  if (c->sch.ck->table) {
+    cplx *ptW = nullptr;
+    if ((rc = dalloc(c, &ptW, W))) return rc;
+    HIPCHK(c, launch_carry_table(c->sch.ck, A.qcW, ptW, c->stream));
+    A.ptW = ptW;
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   // two-level roots of order M
   const uint32_t shift = std::min<uint32_t>(kTwoLevelShift, (uint32_t)log2((double)M));
   const uint32_t S = 1u << shift;

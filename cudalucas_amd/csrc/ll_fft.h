@@ -485,11 +485,10 @@ __device__ __forceinline__ void fft_lds_auto(cplx *buf, const TW &tw, const cplx
   }
 }
 
-// Builds the table fft_lds_auto<..., TAB = true> reads.
-template <int L, int NB, int T, bool TAB, typename TW>
-__device__ __forceinline__ void build_auto_table(cplx *pt, const TW &tw) {
-  if constexpr (TAB && wave_local_ok<L, NB, T>())
-    build_wave_table<L, T, wave_plan_ok<L, NB, T, true>()>(pt, tw);
+// The plan fft_lds_auto's wave-local form uses (uniform radix when valid).
+template <int L, int NB, int T>
+constexpr bool auto_plan_uniform() {
+  return wave_plan_ok<L, NB, T, true>();
 }
 
 // ------------------------------------------------ per-thread twiddle cache

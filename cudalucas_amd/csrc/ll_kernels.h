@@ -33,6 +33,7 @@ struct PlanArgs {
   const double *qcW;       // cos(2 pi e/W), e <= W/4
   const double *qcH;       // cos(2 pi e/H), e <= H/4
   const cplx *ptH;         // column plan's per-pass twiddles, H - 1 roots
+  const cplx *ptW;         // line plan's per-pass twiddles (wave-local carry sweeps)
   TwoLevel twM;            // exp(-2 pi i e/M), e < M
   const double *wA, *iwA;  // 2^(+-fA[a]/n), a < W
   const int32_t *fA;       // (-b * 2H a) mod n
@@ -73,6 +74,7 @@ struct CarryKernel {
   uint32_t w, r;
   const void *carry, *load, *unload;
   uint32_t threads;
+  const void *table;  // builds the line pass table ptW (null: not used)
 };
 
 struct ResidentKernel {
@@ -106,5 +108,7 @@ hipError_t launch_fill(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
 hipError_t launch_copy(const cplx *in, cplx *out, size_t n, hipStream_t s);
 // ptH of a column plan from its quarter-wave table qcH (both device memory)
 hipError_t launch_square_table(const SquareKernel *k, const double *qcH, cplx *ptH, hipStream_t s);
+// ptW of a carry kernel's line plan from qcW (k->table must be set)
+hipError_t launch_carry_table(const CarryKernel *k, const double *qcW, cplx *ptW, hipStream_t s);
 
 }  // namespace llamd

@@ -266,6 +266,11 @@ hipError_t launch_square_table(const SquareKernel *k, const double *qcH, cplx *p
   return hipLaunchKernel(k->table, dim3(1), dim3(256), args, 0, s);
 }
 
+hipError_t launch_carry_table(const CarryKernel *k, const double *qcW, cplx *ptW, hipStream_t s) {
+  void *args[] = {&qcW, &ptW};
+  return hipLaunchKernel(k->table, dim3(1), dim3(256), args, 0, s);
+}
+
 hipError_t launch_copy(const cplx *in, cplx *out, size_t n, hipStream_t s) {
   hipLaunchKernelGGL(copy_stream, dim3(8192), dim3(256), 0, s, in, out, n);
   return hipGetLastError();

@@ -261,8 +261,11 @@ __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict_
   if constexpr (FUSED) fft_lds_cached<H, 2, T, false, 1, R0>(lds, ptw.next);
   else fft_lds_cached<H, 2, T, false>(lds, ptw);
   if (g != 0) {
-    for (int k2 = tid; k2 < H; k2 += T) {
-      const int k2b = H - 1 - k2;
+    // unrolled by 4 so that four iterations' table loads are in flight at once
+    static_assert(H % T == 0, "square sweep: H must be a multiple of T");
+#pragma unroll 4
+    for (int st = 0; st < H / T; st++) {
+      const int k2 = tid + st * T, k2b = H - 1 - k2;
       cplx a = lds[lpad(k2)], b = lds[LP + lpad(k2b)];
       pair_square(a, b, twiddle_m(A.twM, c0 + W * (uint32_t)k2));
       lds[lpad(k2)] = a;
@@ -289,11 +292,18 @@ __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict_
   cplx *out0 = A.x + ((size_t)c0 << rs);
   cplx *out1 = A.x + ((size_t)c1 << rs);
   if constexpr (FUSED) {
+    constexpr int PER = 2 * QL / T;
+    // the last pass's row twiddles w^(c j), loaded while the other passes run
+    cplx tjL[PER];
+#pragma unroll
+    for (int s = 0; s < PER; s++) {
+      const int u = tid + s * T;
+      tjL[s] = twiddle_m(A.twM, cols[u / QL] * (uint32_t)(u % QL));
+    }
     fft_lds_cached<H, 2, T, true, 0, 1, NP - 1>(lds, ptw);
     // last inverse pass: rows j + i QL of each column, straight to HBM
     constexpr int NSL = plan_span(H, NP - 1);
     static_assert(NSL == QL, "last pass spans the whole column");
-    constexpr int PER = 2 * QL / T;
     cplx v[PER][RL];
 #pragma unroll
     for (int s = 0; s < PER; s++) {
@@ -310,7 +320,7 @@ __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict_
       const int u = tid + s * T, b = u / QL, j = u % QL;
       cplx *col = b ? out1 : out0;
       dft<RL, true>(v[s]);
-      const cplx tj = twiddle_m(A.twM, cols[b] * (uint32_t)j);
+      const cplx tj = tjL[s];
 #pragma unroll
       for (int i = 0; i < RL; i++) {
         const int row = j + i * QL;
@@ -341,9 +351,10 @@ __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict_
 // 2^p = 1) after publishing its own; no block waits on block 0 finishing.
 // Should a wait ever outlast kSpinLimit polls, the iteration is reported as
 // failed (error 3.0) rather than hanging the device.
-// The carry sweep keeps per-pass twiddle tables (16 W bytes of LDS) for its
-// wave-local FFT passes when they cost no workgroup per CU (160 KiB of LDS
-// and 2048 threads per CU); the LDS bytes below are the kernel's own arrays.
+// The carry sweep keeps a copy of its line plan's per-pass twiddles (16 W
+// bytes of LDS, in place of the quarter-wave table) for its wave-local FFT
+// passes when that costs no workgroup per CU (160 KiB of LDS and 2048
+// threads per CU); the LDS bytes below are the kernel's own arrays.
 template <int W, int R, int T>
 constexpr bool carry_pass_table() {
   if constexpr (!wave_local_ok<W, R, T>()) {
@@ -351,11 +362,18 @@ constexpr bool carry_pass_table() {
   } else {
     constexpr int kLds = 163840, cap = 2048 / T;
     constexpr int base = R * lpitch(W) * 16 + (W / 4 + 1) * 8 + 2 * R * 4 + 4 * R * 8 + (T / 64) * 4;
-    constexpr int with = base + 16 * W;
+    constexpr int with = base - (W / 4 + 1) * 8 + 16 * W;  // the table replaces qc
     constexpr int a = kLds / base < cap ? kLds / base : cap;
     constexpr int b = kLds / with < cap ? kLds / with : cap;
     return with <= kLds && b >= a;
   }
+}
+
+// The table carry_sweep<W, R, T> reads (W - 1 roots of its line plan, from
+// the quarter-wave table of order W), built once per plan.
+template <int W, int R, int T>
+__global__ __launch_bounds__(256) void line_pass_table(const double *qc, cplx *out) {
+  build_wave_table<W, 256, auto_plan_uniform<W, R, T>()>(out, QuarterRoots<W>{qc});
 }
 
 template <int W, int R, int T>
@@ -365,12 +383,12 @@ __global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArg
   constexpr int LP = lpitch(W);
   constexpr int CPT = (W + T - 1) / T;  // columns per thread
   __shared__ cplx lds[R * LP];
-  __shared__ double qc[W / 4 + 1];
+  __shared__ double qc[carry_pass_table<W, R, T>() ? 1 : W / 4 + 1];
   __shared__ int fb[2 * R];  // weight factors shared by every column
   __shared__ double wb[2 * R], iwb[2 * R];
   __shared__ float red[T / 64];
   constexpr bool TAB = carry_pass_table<W, R, T>();
-  __shared__ cplx ptab[TAB ? W : 1];  // wave-local pass twiddles
+  __shared__ cplx ptab[TAB ? W : 1];  // wave-local pass twiddles (copy of ptW)
   const int tid = threadIdx.x;
   const uint32_t H = A.H;
   const uint32_t G = H / R;
@@ -398,7 +416,11 @@ __global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArg
       const int e = tid + s * T;
       if ((R * W) % T == 0 || e < R * W) v[s] = gload(A.x + (size_t)b0 * W + e);
     }
-    load_quarter<W, T>(qc, A.qcW);
+    if constexpr (TAB) {
+      for (int i = tid; i < W - 1; i += T) ptab[i] = gload(A.ptW + i);
+    } else {
+      load_quarter<W, T>(qc, A.qcW);
+    }
 #pragma unroll
     for (int c = 0; c < (PF ? CPT : 0); c++) {
       const int a = tid + c * T;
@@ -421,7 +443,6 @@ __global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArg
     }
   }
   __syncthreads();
-  build_auto_table<W, R, T, TAB>(ptab, tw);
   fft_lds_auto<W, R, T, true, TAB>(lds, tw, ptab);
 
   // Round, subtract, carry along the 2R consecutive words of each column,
