@@ -155,22 +155,27 @@ __device__ __forceinline__ void pair_square(cplx &A, cplx &B, cplx t) {
 template <int H, int T>
 __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict__ args) {
   const PlanArgs &A = args[blockIdx.y];
-  const bool finished = A.remaining && *A.remaining == 0;  // a batched test that is done
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    // open the iteration for the carry sweep that follows this launch
-    DevState &st = *A.st;
-    if (finished) {
-      st.run = 0;
-    } else {
-      const Minus2 m2 = minus2_position(st.offset, A.p, A.n);
-      st.m2_digit = m2.digit;
-      st.m2_value = m2.value;
-      st.offset = m2.new_offset;
-      st.epoch += 1;
-      st.run = 1;
+  // Opens the iteration for the carry sweep that follows this launch
+  // (workgroup 0) and tells whether this is a batched test that is done.
+  // Called once the column loads are in flight, so that the state reads do
+  // not hold them back.
+  auto open_iteration = [&]() -> bool {
+    const bool finished = A.remaining && *A.remaining == 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      DevState &st = *A.st;
+      if (finished) {
+        st.run = 0;
+      } else {
+        const Minus2 m2 = minus2_position(st.offset, A.p, A.n);
+        st.m2_digit = m2.digit;
+        st.m2_value = m2.value;
+        st.offset = m2.new_offset;
+        st.epoch += 1;
+        st.run = 1;
+      }
     }
-  }
-  if (finished) return;
+    return finished;
+  };
   constexpr int LP = lpitch(H);
   __shared__ cplx lds[2 * LP];
   const int tid = threadIdx.x;
@@ -213,6 +218,7 @@ __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict_
         v[s][i] = gload(col + (((size_t)(row >> rs) * W << rs) | (row & rmask)));
       }
     }
+    if (open_iteration()) return;
     if (tid < 2 * R0) {
       colf0[tid] = twiddle_m(A.twM, cols[tid / R0] * (uint32_t)((tid % R0) * Q0));
     } else if (tid < 2 * (R0 + RL)) {
@@ -243,6 +249,7 @@ __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict_
       v0[st] = gload(col0 + off);
       v1[st] = gload(col1 + off);
     }
+    if (open_iteration()) return;
     const cplx t0 = twiddle_m(A.twM, c0 * (uint32_t)tid), t1 = twiddle_m(A.twM, c1 * (uint32_t)tid);
 #pragma unroll
     for (int st = 0; st < NE; st++) {
@@ -379,7 +386,6 @@ __global__ __launch_bounds__(256) void line_pass_table(const double *qc, cplx *o
 template <int W, int R, int T>
 __global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArgs *__restrict__ args) {
   const PlanArgs &A = args[blockIdx.y];
-  if (!A.st->run) return;  // a finished batched test
   constexpr int LP = lpitch(W);
   constexpr int CPT = (W + T - 1) / T;  // columns per thread
   __shared__ cplx lds[R * LP];
@@ -392,9 +398,6 @@ __global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArg
   const int tid = threadIdx.x;
   const uint32_t H = A.H;
   const uint32_t G = H / R;
-  const uint32_t epoch = A.st->epoch;
-  const uint32_t m2_digit = A.st->m2_digit;
-  const double dmod = (double)A.st->m2_value;
   const uint32_t blk = blockIdx.x;
 
   const uint32_t b0 = blk * R;
@@ -416,6 +419,8 @@ __global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArg
       const int e = tid + s * T;
       if ((R * W) % T == 0 || e < R * W) v[s] = gload(A.x + (size_t)b0 * W + e);
     }
+    // checked once the line loads are in flight (reads only so far)
+    if (!A.st->run) return;  // a finished batched test
     if constexpr (TAB) {
       for (int i = tid; i < W - 1; i += T) ptab[i] = gload(A.ptW + i);
     } else {
@@ -442,6 +447,9 @@ __global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArg
       if ((R * W) % T == 0 || e < R * W) lds[(e % R) * LP + lpad(e / R)] = v[s];
     }
   }
+  const uint32_t epoch = A.st->epoch;
+  const uint32_t m2_digit = A.st->m2_digit;
+  const double dmod = (double)A.st->m2_value;
   __syncthreads();
   fft_lds_auto<W, R, T, true, TAB>(lds, tw, ptab);
 
