@@ -32,6 +32,16 @@ template <typename V>
 __device__ __forceinline__ V gload(const V *p) {
   return *(const __attribute__((address_space(1))) V *)p;
 }
+// Streaming (non-temporal) forms for the signal, which each sweep reads and
+// writes exactly once.
+__device__ __forceinline__ cplx gload_stream(const cplx *p) {
+  const gd2 v = __builtin_nontemporal_load((const __attribute__((address_space(1))) gd2 *)p);
+  return {v.x, v.y};
+}
+__device__ __forceinline__ void gstore_stream(cplx *p, cplx c) {
+  const gd2 v = {c.x, c.y};
+  __builtin_nontemporal_store(v, (__attribute__((address_space(1))) gd2 *)p);
+}
 
 __device__ __forceinline__ cplx cadd(cplx a, cplx b) { return {a.x + b.x, a.y + b.y}; }
 __device__ __forceinline__ cplx csub(cplx a, cplx b) { return {a.x - b.x, a.y - b.y}; }

@@ -215,7 +215,7 @@ __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict_
 #pragma unroll
       for (int i = 0; i < R0; i++) {
         const int row = j + i * Q0;
-        v[s][i] = gload(col + (((size_t)(row >> rs) * W << rs) | (row & rmask)));
+        v[s][i] = gload_stream(col + (((size_t)(row >> rs) * W << rs) | (row & rmask)));
       }
     }
     if (open_iteration()) return;
@@ -246,8 +246,8 @@ __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict_
     for (int st = 0; st < NE; st++) {
       const int i = tid + st * T;
       const size_t off = ((size_t)(i >> rs) * W << rs) | (i & rmask);
-      v0[st] = gload(col0 + off);
-      v1[st] = gload(col1 + off);
+      v0[st] = gload_stream(col0 + off);
+      v1[st] = gload_stream(col1 + off);
     }
     if (open_iteration()) return;
     const cplx t0 = twiddle_m(A.twM, c0 * (uint32_t)tid), t1 = twiddle_m(A.twM, c1 * (uint32_t)tid);
@@ -332,7 +332,7 @@ __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict_
       for (int i = 0; i < RL; i++) {
         const int row = j + i * QL;
         const cplx w = i ? cmul(tj, colfL[b * RL + i]) : tj;
-        gstore(col + (((size_t)(row >> rs) * W << rs) | (row & rmask)), cmulc(v[s][i], w));
+        gstore_stream(col + (((size_t)(row >> rs) * W << rs) | (row & rmask)), cmulc(v[s][i], w));
       }
     }
   } else {
@@ -344,8 +344,8 @@ __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict_
       const size_t off = ((size_t)(i >> rs) * W << rs) | (i & rmask);
       const cplx w0 = cmul(t0, twiddle_m(A.twM, c0 * (uint32_t)(st * T)));
       const cplx w1 = cmul(t1, twiddle_m(A.twM, c1 * (uint32_t)(st * T)));
-      gstore(out0 + off, cmulc(lds[lpad(i)], w0));
-      gstore(out1 + off, cmulc(lds[LP + lpad(i)], w1));
+      gstore_stream(out0 + off, cmulc(lds[lpad(i)], w0));
+      gstore_stream(out1 + off, cmulc(lds[LP + lpad(i)], w1));
     }
   }
 }
@@ -417,7 +417,7 @@ __global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArg
 #pragma unroll
     for (int s = 0; s < NE; s++) {
       const int e = tid + s * T;
-      if ((R * W) % T == 0 || e < R * W) v[s] = gload(A.x + (size_t)b0 * W + e);
+      if ((R * W) % T == 0 || e < R * W) v[s] = gload_stream(A.x + (size_t)b0 * W + e);
     }
     // checked once the line loads are in flight (reads only so far)
     if (!A.st->run) return;  // a finished batched test
@@ -564,7 +564,7 @@ __global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArg
 #pragma unroll
     for (int s = 0; s < NE; s++) {
       const int e = tid + s * T;
-      if ((R * W) % T == 0 || e < R * W) gstore(A.x + (size_t)b0 * W + e, v[s]);
+      if ((R * W) % T == 0 || e < R * W) gstore_stream(A.x + (size_t)b0 * W + e, v[s]);
     }
   }
   const float m = block_max<T>((float)emax, red);
