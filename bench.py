@@ -109,6 +109,9 @@ def batched_leg(steps=4, step_iters=512, plain_iters=200):
             c.iterate(plain_iters)
             per.append((p - 2) * c.last_iterate_ms() * 1e-3 / plain_iters)
     plain = 3600.0 / float(np.mean(per))
+    # algorithmic bytes of one batch iteration: 32 N per test (two read+write
+    # sweeps of its N-word FP64 signal; LDS-resident tests counted alike)
+    b_alg = 32.0 * float(sum(lens))
     return {
         "workload": "256 seeded primes log-uniform in [1e5, 5e6], FFT lengths %dK-%dK (%d schedule groups)"
         % (min(lens) // 1024, max(lens) // 1024, groups),
@@ -116,6 +119,7 @@ def batched_leg(steps=4, step_iters=512, plain_iters=200):
         "exps_per_hour_plain": round(plain, 1),
         "speedup": round(batched / plain, 2),
         "batch_iteration_ms": round(t_step * 1e3, 4),
+        "hbm_gbs_alg": round(b_alg / t_step / 1e9, 1),
         "sample": "%d timed batch iterations after 64 warm-up; plain: %d iterations per exponent, all 256"
         % (steps * step_iters, plain_iters),
         "schedules": "batched autotuner table" if tuned else "built-in preference",

@@ -637,8 +637,9 @@ namespace {
 constexpr uint32_t kBatchRing = 1024;  // iterations per batch step at most
 // Two-sweep groups are spread over this many streams (lanes), so that the
 // launches of different groups overlap instead of queueing one behind the
-// other; with the resident stream that makes four hardware queues.
-constexpr uint32_t kSweepLanes = 3;
+// other (measured on the 256-exponent batch: 2 lanes 0.43 ms per batch
+// iteration, 3 lanes 0.53, 4 lanes 0.49, 6 lanes 0.53, 1 lane 0.74).
+constexpr uint32_t kSweepLanes = 2;
 
 struct BatchGroup {
   Schedule sch{};
