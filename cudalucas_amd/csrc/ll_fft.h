@@ -481,8 +481,8 @@ __device__ __forceinline__ void fft_lds_wave(cplx *buf, const TW &tw) {
 }
 
 // Either form, chosen at compile time.  With TAB the wave-local form takes
-// its twiddles from the per-pass table `pt` (build_auto_table, L - 1
-// entries, built before the first transform), else from `tw`.
+// its twiddles from the per-pass table `pt` (build_wave_table of the plan
+// auto_plan_uniform selects, L - 1 entries), else from `tw`.
 template <int L, int NB, int T, bool INV, bool TAB, typename TW>
 __device__ __forceinline__ void fft_lds_auto(cplx *buf, const TW &tw, const cplx *pt) {
   if constexpr (wave_local_ok<L, NB, T>()) {
@@ -504,16 +504,15 @@ constexpr bool auto_plan_uniform() {
 // ------------------------------------------------ per-thread twiddle cache
 // In pass PASS of an NB x L Stockham transform run by T threads, thread t
 // always handles butterflies u = t + s*T, so its twiddles w^(i*(j mod NS))
-// are fixed: they are computed once per kernel and shared by the forward
-// and the inverse transform (the inverse uses their conjugates), which
-// takes the table lookups out of the passes.
+// are fixed: they are loaded once per kernel from the plan's per-pass table
+// (build_wave_table of the default plan, in global memory) and shared by
+// the forward and the inverse transform (the inverse uses their
+// conjugates), which takes the table lookups out of the passes.
 template <int L, int NB, int T, int PASS = 0, int NS = 1, bool END = (NS >= L)>
 struct PassTwiddles;
 
 template <int L, int NB, int T, int PASS, int NS>
 struct PassTwiddles<L, NB, T, PASS, NS, true> {
-  template <typename TW>
-  __device__ __forceinline__ void init(const TW &) {}
   __device__ __forceinline__ void init_table(const cplx *) {}
 };
 
@@ -527,22 +526,6 @@ struct PassTwiddles<L, NB, T, PASS, NS, false> {
   cplx w[PER][NW];
   PassTwiddles<L, NB, T, PASS + 1, NS * R> next;
 
-  template <typename TW>
-  __device__ __forceinline__ void init(const TW &tw) {
-    if constexpr (NS > 1) {
-#pragma unroll
-      for (int s = 0; s < PER; s++) {
-        const int u = (int)threadIdx.x + s * T;
-        const int k = (u % Q) % NS;
-#pragma unroll
-        for (int i = 1; i < R; i++) w[s][i - 1] = tw((i * k) * (L / (NS * R)));
-      }
-    }
-    next.init(tw);
-  }
-
-  // The same twiddles read from the plan's per-pass table (build_wave_table
-  // of the default plan, in global memory).
   __device__ __forceinline__ void init_table(const cplx *__restrict__ pt) {
     if constexpr (NS > 1) {
 #pragma unroll
