@@ -635,16 +635,10 @@ int ll_debug_inject(ll_ctx *c, uint32_t kind, uint64_t after, float value) {
 namespace {
 
 constexpr uint32_t kBatchRing = 1024;  // iterations per batch step at most
-// Two-sweep groups are spread over this many streams (lanes), so that the
-// launches of different groups overlap instead of queueing one behind the
-// other (measured on the 256-exponent batch: 2 lanes 0.43 ms per batch
-// iteration, 3 lanes 0.53, 4 lanes 0.49, 6 lanes 0.53, 1 lane 0.74).
-constexpr uint32_t kSweepLanes = 2;
 
 struct BatchGroup {
   Schedule sch{};
   uint32_t n = 0;
-  uint32_t lane = 0;              // sweeps: stream index (0: the batch stream)
   std::vector<uint32_t> members;  // test indices
   PlanArgs *d_args = nullptr;     // sweeps: per-member args
   ResidentDesc *d_desc = nullptr; // resident: per-member descriptors
@@ -657,8 +651,6 @@ struct ll_batch {
   int device = 0;
   hipStream_t stream = nullptr, stream2 = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  hipStream_t lanes[kSweepLanes] = {};      // lanes[0] = stream
-  hipEvent_t lane_ev[kSweepLanes + 1] = {};  // fork / join of the lanes
   std::vector<ll_ctx *> tests;
   std::vector<BatchGroup> groups;
   std::vector<void *> allocs;
@@ -694,10 +686,6 @@ static void batch_release(ll_batch *b) {
   b->allocs.clear();
   for (auto &e : b->ev)
     if (e) (void)hipEventDestroy(e);
-  for (auto &e : b->lane_ev)
-    if (e) (void)hipEventDestroy(e);
-  for (uint32_t l = 1; l < kSweepLanes; l++)
-    if (b->lanes[l]) (void)hipStreamDestroy(b->lanes[l]);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   if (b->stream2) (void)hipStreamDestroy(b->stream2);
 }
@@ -711,23 +699,19 @@ static int balloc(ll_batch *b, T **ptr, size_t count) {
   return LL_OK;
 }
 
-// `iters` iterations of every two-sweep group, each group on its lane: the
-// lanes start after the work already queued on the batch stream and the
-// batch stream continues after all of them.
+// `iters` iterations of every two-sweep group on the batch stream.  The
+// groups' launches run one after another: a carry sweep waits on other
+// workgroups of its own launch, which is safe for one launch at a time
+// (each XCD dispatches its workgroups in order) but not for two carry
+// launches sharing the GPU, which can hold the slots each other's awaited
+// workgroups need.
 static int enqueue_sweeps(ll_batch *b, uint32_t iters) {
-  BCHK(b, hipEventRecord(b->lane_ev[0], b->stream));
-  for (uint32_t l = 1; l < kSweepLanes; l++) BCHK(b, hipStreamWaitEvent(b->lanes[l], b->lane_ev[0], 0));
   for (uint32_t i = 0; i < iters; i++)
     for (auto &grp : b->groups)
       if (grp.sch.path == 1) {
-        hipStream_t s = b->lanes[grp.lane];
-        BCHK(b, launch_square(grp.sch.sk, grp.d_args, (uint32_t)grp.members.size(), grp.sch.W, s));
-        BCHK(b, launch_carry(grp.sch.ck, grp.d_args, (uint32_t)grp.members.size(), grp.sch.H, s));
+        BCHK(b, launch_square(grp.sch.sk, grp.d_args, (uint32_t)grp.members.size(), grp.sch.W, b->stream));
+        BCHK(b, launch_carry(grp.sch.ck, grp.d_args, (uint32_t)grp.members.size(), grp.sch.H, b->stream));
       }
-  for (uint32_t l = 1; l < kSweepLanes; l++) {
-    BCHK(b, hipEventRecord(b->lane_ev[l], b->lanes[l]));
-    BCHK(b, hipStreamWaitEvent(b->stream, b->lane_ev[l], 0));
-  }
   return LL_OK;
 }
 
@@ -762,9 +746,6 @@ int ll_batch_create(int32_t device, uint32_t count, const ll_batch_test *tests, 
   if (hipSetDevice(device) != hipSuccess) return bfail(b, LL_ERR_HIP, "hipSetDevice(%d)", device);
   BCHK(b, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
   BCHK(b, hipStreamCreateWithFlags(&b->stream2, hipStreamNonBlocking));
-  b->lanes[0] = b->stream;
-  for (uint32_t l = 1; l < kSweepLanes; l++) BCHK(b, hipStreamCreateWithFlags(&b->lanes[l], hipStreamNonBlocking));
-  for (auto &e : b->lane_ev) BCHK(b, hipEventCreateWithFlags(&e, hipEventDisableTiming));
   for (auto &e : b->ev) BCHK(b, hipEventCreate(&e));
   int rc = balloc(b, &b->d_remaining, count);
   if (rc) return rc;
@@ -818,22 +799,6 @@ int ll_batch_create(int32_t device, uint32_t count, const ll_batch_test *tests, 
       if ((rc = balloc(b, &g.d_desc, k))) return rc;
     }
   }
-  // spread the two-sweep groups over the lanes, largest first, each to the
-  // lane with the least work so far (work ~ words x tests)
-  std::vector<BatchGroup *> order;
-  for (auto &g : b->groups)
-    if (g.sch.path == 1) order.push_back(&g);
-  std::sort(order.begin(), order.end(), [](const BatchGroup *x, const BatchGroup *y) {
-    return (uint64_t)x->n * x->members.size() > (uint64_t)y->n * y->members.size();
-  });
-  uint64_t load[kSweepLanes] = {};
-  for (BatchGroup *g : order) {
-    uint32_t best = 0;
-    for (uint32_t l = 1; l < kSweepLanes; l++)
-      if (load[l] < load[best]) best = l;
-    g->lane = best;
-    load[best] += (uint64_t)g->n * g->members.size();
-  }
   return LL_OK;
 }
 
@@ -841,8 +806,6 @@ void ll_batch_destroy(ll_batch *b) {
   if (!b) return;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   if (b->stream2) (void)hipStreamSynchronize(b->stream2);
-  for (uint32_t l = 1; l < kSweepLanes; l++)
-    if (b->lanes[l]) (void)hipStreamSynchronize(b->lanes[l]);
   batch_release(b);
   delete b;
 }

@@ -12,6 +12,13 @@
  * sum_j d_j 2^ceil(p j / n) mod 2^p - 1 of the *shifted* residue
  * s * 2^offset, exactly as the reference's x_int / g_xint arrays
  * (CUDALucas.cu:1254-1342).
+ *
+ * Concurrency: a context (or batch) runs its kernels on its own stream(s).
+ * The carry sweep of the two-sweep path waits on other workgroups of the
+ * same launch; two such launches sharing one GPU can hold each other's
+ * workgroup slots, so iterate at most one two-sweep context or batch per
+ * GPU at a time (a batch orders its own carry sweeps itself).  A wait that
+ * does not end is reported as a failed iteration (error 3.0), never a hang.
  */
 #ifndef LL_API_H
 #define LL_API_H
