@@ -28,9 +28,12 @@
 // cyclic wrap (the last block's carries feed block 0, one column to the
 // left) is the one wait on a later block, and it cannot deadlock because
 // every block publishes its own carries before it waits, so the last block
-// never waits on block 0.  The square sweep's workgroup 0 opens each
-// iteration in a small device state (epoch, offset, -2 position) that the
-// carry sweep reads.
+// never waits on block 0.  That argument holds for one carry launch at a
+// time on the GPU (each XCD dispatches its workgroups in order); two carry
+// launches running side by side can hold each other's workgroup slots, so
+// callers run them one after another.  The square sweep's workgroup 0 opens
+// each iteration in a small device state (epoch, offset, -2 position) that
+// the carry sweep reads.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
