@@ -78,3 +78,37 @@ def test_batch_forced_sweeps_small_lengths():
         b.run(max_step=256)
         for i, (p, n, off, it) in enumerate(tests):
             assert (b.residue(i)[0] == O.gmp_ll(p, it)).all()
+
+
+def test_large_multi_group_batch_has_no_failed_iterations():
+    """A batch of several large two-sweep groups (256K-1M words, 8 tests
+    each, the shape of the round-off judge's batches): every iteration's
+    error stays far below the limit, so no carry wait timed out (those
+    report 3.0), and residues equal the plain engine's.  Regression test
+    for carry launches of different groups sharing the GPU."""
+    rng = W.SplitMix64(2024)
+    lengths = [262144, 524288, 1048576]
+    tests = []
+    for n in lengths:
+        for _ in range(8):
+            p = W.next_prime(int(n * (14.0 + (rng.next() % 1000) / 250.0)))
+            tests.append((p, n, rng.next() % p, 600))
+    announce("large multi-group batch %d tests" % len(tests))
+    with L.LLBatch(tests) as b:
+        assert b.groups() == len(lengths)
+        errs = [[] for _ in tests]
+        for k in (512, 88):
+            b.step(k)
+            for i in range(len(tests)):
+                errs[i].append(b.step_errors(i))
+        res = [b.residue(i)[0] for i in range(len(tests))]
+    for i, (p, n, off, it) in enumerate(tests):
+        e = np.concatenate(errs[i])
+        assert len(e) == it and e.max() < 0.35, "p=%d n=%d max %.3f" % (p, n, e.max())
+    for i in (0, 8, 16):  # one per group against the plain engine
+        p, n, off, it = tests[i]
+        with L.LLContext(p, n) as c:
+            c.load_initial(off)
+            c.iterate(it)
+            wp, _ = c.residue()
+        assert (res[i] == wp).all(), "p=%d n=%d" % (p, n)
