@@ -148,8 +148,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; the LL path of one exponent runs on a "
-              "single MI355X (multi-GPU is sharding by exponent, see DESIGN.md)", file=sys.stderr)
+        print("bench.py: measures one GPU; the LL path of one exponent runs on a single "
+              "MI355X, and several GPUs would each take their own exponents", file=sys.stderr)
         sys.exit(2)
 
     import cudalucas_amd as L

@@ -168,8 +168,8 @@ def test_initial_digits_value():
 
 
 def test_fft_bounds_reference_examples():
-    # choose_fft_length's power laws (CUDALucas.cu:1034-1050): the survey's
-    # worked values -- 77M needs more than 4M words, 4096K ends near 75.8M.
+    # choose_fft_length's power laws (CUDALucas.cu:1034-1050), worked by
+    # hand: 77M needs more than 4M words, 4096K ends near 75.8M.
     lb, ub = O.fft_bounds(77_000_000)
     assert lb == 4216441 and lb > 4194304
     lens = O.default_fft_lengths()
