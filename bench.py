@@ -69,7 +69,7 @@ def cpu_baseline(p, n, budget_s):
 
 
 def batched_leg(steps=4, step_iters=512, plain_iters=200):
-    """C4 workload: 256 seeded exponents log-uniform in [1e5, 5e6] at their
+    """Batched workload: 256 seeded exponents log-uniform in [1e5, 5e6] at their
     chosen FFT lengths.  Exponents/hour, with finished tests replaced by new
     ones of the same mix: batched = 3600 * 256 / (t_step * mean(p - 2));
     plain (one exponent at a time on the same GPU) =
@@ -80,7 +80,7 @@ def batched_leg(steps=4, step_iters=512, plain_iters=200):
     from cudalucas_amd import tuner as T
     from cudalucas_amd import workloads as W
 
-    ps = W.log_uniform_primes(W.C4_SEED, W.C4_COUNT, *W.C4_RANGE)
+    ps = W.log_uniform_primes(W.BATCH_SEED, W.BATCH_COUNT, *W.BATCH_RANGE)
     lens = [L.choose_fft_length(p)[0] for p in ps]
     tests = [(p, n, p % 1000, p - 2) for p, n in zip(ps, lens)]
     tuned = os.path.exists(T.BATCH_TABLE)
@@ -164,7 +164,7 @@ def main():
 
     from cudalucas_amd import tuner as T
 
-    p, n = W.C2_P, W.C2_N
+    p, n = W.LARGE_P, W.LARGE_N
     single_tuned = os.path.exists(T.SINGLE_TABLE)
     if single_tuned:  # schedules picked by the single-exponent autotuner
         T.install(T.load(T.SINGLE_TABLE))

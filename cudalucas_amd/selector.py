@@ -91,7 +91,7 @@ def candidates(p):
 
 # ---------------------------------------------------------------- traces
 
-def trace_plan(count=W.C5_TRAIN_COUNT, seed=W.C5_TRAIN_SEED, lo=W.C5_TRAIN_RANGE[0], hi=W.C5_TRAIN_RANGE[1]):
+def trace_plan(count=W.SELECTOR_TRAIN_COUNT, seed=W.SELECTOR_TRAIN_SEED, lo=W.SELECTOR_TRAIN_RANGE[0], hi=W.SELECTOR_TRAIN_RANGE[1]):
     """[(p, n, offset)]: seeded log-uniform primes, a uniform candidate n."""
     ps = W.log_uniform_primes(seed, count, lo, hi)
     rng = W.SplitMix64(seed ^ 0x5E1EC7)
@@ -477,7 +477,7 @@ def _batch_iteration_ms(tests, iters, words_budget=1 << 27, max_tests=256):
 
 
 def evaluate(model_path, primes, times, batch_times=None, schedules=None, judge_iters=JUDGE_ITERS,
-             batch_iters=512, seed=W.C5_EVAL_SEED, log=None):
+             batch_iters=512, seed=W.SELECTOR_EVAL_SEED, log=None):
     """The learned selector against the reference policy on `primes`.
 
     Two uses of the same model, each through the round-off judge with
@@ -577,7 +577,7 @@ def main(argv=None):
     sub = ap.add_subparsers(dest="cmd", required=True)
     a = sub.add_parser("traces")
     a.add_argument("--out", required=True)
-    a.add_argument("--count", type=int, default=W.C5_TRAIN_COUNT)
+    a.add_argument("--count", type=int, default=W.SELECTOR_TRAIN_COUNT)
     a = sub.add_parser("train")
     a.add_argument("--traces", required=True)
     a.add_argument("--model", default=MODEL_PATH)
@@ -590,7 +590,7 @@ def main(argv=None):
     a.add_argument("--times", default=TIMES_PATH)
     a.add_argument("--batch-times", default=BATCH_TIMES_PATH)
     a.add_argument("--schedules", default=None)
-    a.add_argument("--count", type=int, default=W.C5_EVAL_COUNT)
+    a.add_argument("--count", type=int, default=W.SELECTOR_EVAL_COUNT)
     a.add_argument("--out")
     args = ap.parse_args(argv)
     if args.cmd == "traces":
@@ -605,7 +605,7 @@ def main(argv=None):
         d = np.load(args.traces)
         model, stats = fit(d["p"], d["n"], d["window_max"], log=_log)
         save_model(model, args.model, comment="trained on %d (p, n) roundoff traces, p log-uniform in "
-                   "[1e5, 1.2e8] (seed %d)" % (stats["samples"], W.C5_TRAIN_SEED))
+                   "[1e5, 1.2e8] (seed %d)" % (stats["samples"], W.SELECTOR_TRAIN_SEED))
         _log(json.dumps(stats))
         if args.stats:
             with open(args.stats, "w") as f:
@@ -614,7 +614,7 @@ def main(argv=None):
         save_times(bench_lengths(supported_lengths(), args.iters, log=_log), args.out)
     elif args.cmd == "evaluate":
         times = read_times(args.times)
-        primes = W.log_uniform_primes(W.C5_EVAL_SEED, args.count, *W.C5_EVAL_RANGE)
+        primes = W.log_uniform_primes(W.SELECTOR_EVAL_SEED, args.count, *W.SELECTOR_EVAL_RANGE)
         from . import tuner
 
         btimes = read_times(args.batch_times) if args.batch_times and os.path.exists(args.batch_times) else None

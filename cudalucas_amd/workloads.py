@@ -74,42 +74,42 @@ def log_uniform_primes(seed, count, lo, hi):
 
 # Plain LL test, single exponent: seeded prime in [40000, 50000], N = 4096,
 # all p-2 iterations; 44497 (a Mersenne-prime exponent) ends at residue 0.
-C1_SEED = 1001
-C1_N = 4096
-C1_P = seeded_prime(C1_SEED, 40000, 50000)
-C1_PRIME_P = 44497
+SMALL_SEED = 1001
+SMALL_N = 4096
+SMALL_P = seeded_prime(SMALL_SEED, 40000, 50000)
+SMALL_PRIME_P = 44497
 
 # Large single exponent: seeded prime near 77,000,000 at N = 4,194,304.
-C2_SEED = 2002
-C2_N = 4194304
-C2_P = seeded_prime(C2_SEED, 77_000_000, 77_100_000)
-C2_TIMED_ITERS = 10_000
-C2_CHECK_ITER = 1000
+LARGE_SEED = 2002
+LARGE_N = 4194304
+LARGE_P = seeded_prime(LARGE_SEED, 77_000_000, 77_100_000)
+LARGE_TIMED_ITERS = 10_000
+LARGE_CHECK_ITER = 1000
 # Reference self-test exponent in the same FFT class (CUDALucas.cu:2705):
 # Res64 after 10,000 iterations.
-C2_REF_P = 75_846_319
-C2_REF_RES64 = 0x626AB6B09B607BA9
+LARGE_REF_P = 75_846_319
+LARGE_REF_RES64 = 0x626AB6B09B607BA9
 
 # Smooth-length sweep: p at ~18.5 bits/word for 3*2^k, 5*2^k, 7*2^k lengths.
-C3_SEED = 3003
-C3_LENGTHS = [1572864, 2621440, 3670016, 5242880, 6291456]
+SMOOTH_SEED = 3003
+SMOOTH_LENGTHS = [1572864, 2621440, 3670016, 5242880, 6291456]
 
 
-def c3_prime(n, seed=C3_SEED, bits_per_word=18.5):
+def smooth_prime(n, seed=SMOOTH_SEED, bits_per_word=18.5):
     target = int(bits_per_word * n)
     return seeded_prime(seed + n, target - 1000, target)
 
 
 # Batched: 256 exponents log-uniform in [1e5, 5e6].
-C4_SEED = 4004
-C4_COUNT = 256
-C4_RANGE = (100_000, 5_000_000)
+BATCH_SEED = 4004
+BATCH_COUNT = 256
+BATCH_RANGE = (100_000, 5_000_000)
 
 # Learned selector: training p log-uniform in [1e5, 1.2e8]; evaluation on
 # 512 seeded primes in [1e5, 2e7].
-C5_TRAIN_SEED = 5005
-C5_EVAL_SEED = 5006
-C5_EVAL_COUNT = 512
-C5_EVAL_RANGE = (100_000, 20_000_000)
-C5_TRAIN_COUNT = 20_000
-C5_TRAIN_RANGE = (100_000, 120_000_000)
+SELECTOR_TRAIN_SEED = 5005
+SELECTOR_EVAL_SEED = 5006
+SELECTOR_EVAL_COUNT = 512
+SELECTOR_EVAL_RANGE = (100_000, 20_000_000)
+SELECTOR_TRAIN_COUNT = 20_000
+SELECTOR_TRAIN_RANGE = (100_000, 120_000_000)

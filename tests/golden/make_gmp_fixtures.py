@@ -46,10 +46,10 @@ def main():
             data = json.load(f)
     jobs = []
     # smooth-length sweep exponents: prefix of 200 iterations
-    for n in W.C3_LENGTHS:
-        jobs.append((W.c3_prime(n), [200]))
+    for n in W.SMOOTH_LENGTHS:
+        jobs.append((W.smooth_prime(n), [200]))
     # large single exponent: checkpoints every 100 iterations up to 1000
-    jobs.append((W.C2_P, list(range(100, W.C2_CHECK_ITER + 1, 100))))
+    jobs.append((W.LARGE_P, list(range(100, W.LARGE_CHECK_ITER + 1, 100))))
     # two-sweep schedule cases of tests/test_gpu_parity.py above 5e6
     for p in SWEEP_FIXTURE_PRIMES:
         jobs.append((p, [SWEEP_ITERS]))

@@ -2,7 +2,8 @@
 
 Every residue is checked bit-exactly: against GMP where an exact run is
 cheap, and against the plain single-exponent engine (itself pinned to GMP
-and the reference's Res64 tables) for the seeded batch of the C4 workload.
+and the reference's Res64 tables) for a sample of the seeded 256-exponent
+batch workload.
 """
 import numpy as np
 import pytest
@@ -19,7 +20,7 @@ def test_mixed_batch_runs_to_completion_vs_gmp():
     tests = [
         (44497, 4096, 17, 44497 - 2),       # Mersenne prime: residue 0
         (21701, 2048, 5, 21701 - 2),        # Mersenne prime: residue 0
-        (W.C1_P, 4096, 77, W.C1_P - 2),     # full composite test
+        (W.SMALL_P, 4096, 77, W.SMALL_P - 2),     # full composite test
         (86243, 8192, 3, 3000),
         (1257787, 131072, 11, 1500),        # two-sweep group (256 x 256)
         (2553659, 131072, 12, 1500),
@@ -42,8 +43,8 @@ def test_mixed_batch_runs_to_completion_vs_gmp():
         assert [b.residue(i)[1] for i in range(len(tests))] == before
 
 
-def c4_sample(k):
-    ps = W.log_uniform_primes(W.C4_SEED, W.C4_COUNT, *W.C4_RANGE)
+def batch_sample(k):
+    ps = W.log_uniform_primes(W.BATCH_SEED, W.BATCH_COUNT, *W.BATCH_RANGE)
     idx = np.linspace(0, len(ps) - 1, k).astype(int)
     out = []
     for j in idx:
@@ -54,10 +55,10 @@ def c4_sample(k):
 
 
 def test_seeded_batch_matches_plain_engine():
-    sample = c4_sample(12)
+    sample = batch_sample(12)
     iters = 1500
     tests = [(p, n, (p * 7) % p, iters) for p, n in sample]
-    announce("batch C4 sample %s" % [(p, n) for p, n in sample])
+    announce("batch workload sample %s" % [(p, n) for p, n in sample])
     with L.LLBatch(tests) as b:
         b.step(1000)
         b.step(500)

@@ -11,10 +11,10 @@ final residue equals the plain single-exponent engine's, run to completion
 beside it.  The run takes about 40 GPU-minutes, so it is split in four
 parts and only runs when asked:
 
-    LL_C4_COMPLETION=low  python -m pytest -s tests/test_gpu_c4_completion.py   # p < 2e6
-    LL_C4_COMPLETION=mid  python -m pytest -s tests/test_gpu_c4_completion.py   # 2e6 <= p < 3.5e6
-    LL_C4_COMPLETION=high python -m pytest -s tests/test_gpu_c4_completion.py   # 3.5e6 <= p < 4.3e6
-    LL_C4_COMPLETION=top  python -m pytest -s tests/test_gpu_c4_completion.py   # p >= 4.3e6
+    LL_BATCH_COMPLETION=low  python -m pytest -s tests/test_gpu_batch_completion.py   # p < 2e6
+    LL_BATCH_COMPLETION=mid  python -m pytest -s tests/test_gpu_batch_completion.py   # 2e6 <= p < 3.5e6
+    LL_BATCH_COMPLETION=high python -m pytest -s tests/test_gpu_batch_completion.py   # 3.5e6 <= p < 4.3e6
+    LL_BATCH_COMPLETION=top  python -m pytest -s tests/test_gpu_batch_completion.py   # p >= 4.3e6
 """
 import hashlib
 import json
@@ -35,21 +35,21 @@ MERSENNE_PRIME_EXPONENTS = [110503, 132049, 216091, 756839, 859433, 1257787, 139
 PARTS = {"low": (0, 2_000_000), "mid": (2_000_000, 3_500_000), "high": (3_500_000, 4_300_000),
          "top": (4_300_000, 10_000_000)}
 PLAIN_CHECKS = 24  # smallest exponents also run to completion in the plain engine
-PART = os.environ.get("LL_C4_COMPLETION", "")
+PART = os.environ.get("LL_BATCH_COMPLETION", "")
 
 
 def digest(words):
     return hashlib.sha256(np.ascontiguousarray(words, dtype="<u4").tobytes()).hexdigest()[:32]
 
 
-@pytest.mark.skipif(PART not in PARTS, reason="set LL_C4_COMPLETION=low|mid|high|top (8-12 GPU-minutes each)")
-def test_c4_batch_to_completion():
-    ps = W.log_uniform_primes(W.C4_SEED, W.C4_COUNT, *W.C4_RANGE)
+@pytest.mark.skipif(PART not in PARTS, reason="set LL_BATCH_COMPLETION=low|mid|high|top (8-12 GPU-minutes each)")
+def test_batch_to_completion():
+    ps = W.log_uniform_primes(W.BATCH_SEED, W.BATCH_COUNT, *W.BATCH_RANGE)
     ps = sorted(set(ps) | set(MERSENNE_PRIME_EXPONENTS))
     lo, hi = PARTS[PART]
     ps = [p for p in ps if lo <= p < hi]
     tests = [(p, L.choose_fft_length(p)[0], p % 1000, p - 2) for p in ps]
-    announce("C4 completion part %s: %d exponents, %d..%d" % (PART, len(ps), ps[0], ps[-1]))
+    announce("batch completion part %s: %d exponents, %d..%d" % (PART, len(ps), ps[0], ps[-1]))
     if os.path.exists(T.BATCH_TABLE):
         T.install(T.load(T.BATCH_TABLE))
     t0 = time.time()
@@ -82,7 +82,7 @@ def test_c4_batch_to_completion():
             m = c.iterate(p - 2)
             w, r = c.residue()
         plain[p] = digest(w)
-    out = os.environ.get("LL_C4_REPORT")
+    out = os.environ.get("LL_BATCH_REPORT")
     if out:
         with open(out, "w") as f:
             json.dump({"part": PART, "wall_s": wall, "rows": rows, "plain_digests": plain}, f, indent=1)

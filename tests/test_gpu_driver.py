@@ -48,7 +48,7 @@ def test_prime_exponent_reports_prime(tmp_path):
 
 
 def test_composite_run_matches_gmp_and_reports(tmp_path):
-    p = W.C1_P
+    p = W.SMALL_P
     r = drive(["-f", "4096", "-x", "10000", "-c", "10000", "--offset", "123", p], tmp_path)
     assert r.returncode == 0, r.stdout + r.stderr
     want = gmp_res64(p, p - 2)
@@ -63,7 +63,7 @@ def test_composite_run_matches_gmp_and_reports(tmp_path):
 
 
 def test_stop_and_resume_from_own_checkpoint(tmp_path):
-    p = W.C1_P
+    p = W.SMALL_P
     r1 = drive(["-f", "4096", "-x", "5000", "-c", "5000", "--offset", "7", "--stop-after", "12000", p], tmp_path)
     assert r1.returncode == 0, r1.stdout + r1.stderr
     assert (tmp_path / ("c%d" % p)).exists()
@@ -77,7 +77,7 @@ def test_resume_from_reference_format_checkpoint(tmp_path):
     """A c<p> file laid out as CUDALucas writes it (CUDALucas.cu:1707-1849),
     built by the oracle's restatement of pack_bits and the CRC from the exact
     GMP state after 20,000 iterations, shift offset 0, FFT length 4096."""
-    p, n, done = W.C1_P, 4096, 20000
+    p, n, done = W.SMALL_P, 4096, 20000
     s = O.gmp_ll(p, done)
     end = O.nwords(p)
     words = np.zeros(end + 10, dtype=np.uint32)
@@ -93,7 +93,7 @@ def test_resume_from_reference_format_checkpoint(tmp_path):
 
 
 def test_roundoff_error_retry_then_larger_fft(tmp_path):
-    p = W.C1_P
+    p = W.SMALL_P
     # once: retried from the last report state and the error goes away
     r = drive(["-f", "4096", "-x", "5000", "--offset", "5", "--inject-error", "7001:0.45:1", p], tmp_path)
     assert r.returncode == 0, r.stdout + r.stderr
@@ -122,11 +122,11 @@ def test_selftest_short_table(tmp_path):
 def test_worktodo_and_results(tmp_path):
     aid = "0123456789ABCDEF0123456789ABCDEF"
     (tmp_path / "worktodo.txt").write_text(
-        "Test=N/A,%d,69,0\n# comment\nDoubleCheck=%s,44497,68,1\n" % (W.C1_P, aid))
+        "Test=N/A,%d,69,0\n# comment\nDoubleCheck=%s,44497,68,1\n" % (W.SMALL_P, aid))
     r = drive(["-x", "10000"], tmp_path)
     assert r.returncode == 2  # stops when no assignment is left
     lines = (tmp_path / "results.txt").read_text().strip().splitlines()
-    assert lines[0].startswith("M( %d )C, 0x%016x" % (W.C1_P, gmp_res64(W.C1_P, W.C1_P - 2)))
+    assert lines[0].startswith("M( %d )C, 0x%016x" % (W.SMALL_P, gmp_res64(W.SMALL_P, W.SMALL_P - 2)))
     assert lines[1].startswith("M( 44497 )P") and lines[1].endswith("AID: " + aid)
     assert (tmp_path / "worktodo.txt").read_text() == "# comment\n"
 

@@ -45,8 +45,8 @@ def rotl(words, p, k):
 # ----------------------------------------------------------- small N (LDS)
 
 @pytest.mark.parametrize("p,n,iters,offset", [
-    (W.C1_P, 4096, 500, 0),
-    (W.C1_P, 4096, 500, 12345),
+    (W.SMALL_P, 4096, 500, 0),
+    (W.SMALL_P, 4096, 500, 12345),
     (44497, 4096, 300, 777),
     (22133, 1024, 1000, 5),
     (86243, 8192, 400, 99),
@@ -62,9 +62,9 @@ def test_resident_vs_gmp(p, n, iters, offset):
     assert m < ROUNDOFF_LIMIT
 
 
-def test_c1_full_run_vs_gmp():
-    p, n = W.C1_P, W.C1_N
-    announce("C1 full run p=%d n=%d" % (p, n))
+def test_small_exponent_full_run_vs_gmp():
+    p, n = W.SMALL_P, W.SMALL_N
+    announce("small exponent full run p=%d n=%d" % (p, n))
     with L.LLContext(p, n) as c:
         c.load_initial(0)
         m, errs = c.iterate(p - 2, per_iter=True)
@@ -77,7 +77,7 @@ def test_c1_full_run_vs_gmp():
 
 
 def test_mersenne_prime_44497_is_zero():
-    p, n = W.C1_PRIME_P, 4096
+    p, n = W.SMALL_PRIME_P, 4096
     announce("M44497 full run")
     w, r, m = L.run_ll(p, n, p - 2, offset=31337)
     assert r == 0 and not w.any()
@@ -186,12 +186,12 @@ def test_reference_selftest_res64(p, res64, n):
 # ---------------------------------------------------- large configurations
 
 @pytest.mark.parametrize("schedule", [None, (512, 4096, 8)])
-def test_c2_77m_vs_gmp_fixture(gmp_fixtures, schedule):
+def test_77m_at_4m_words_vs_gmp_fixture(gmp_fixtures, schedule):
     """The built-in schedule and the autotuned one (the bench's)."""
-    p, n = W.C2_P, W.C2_N
+    p, n = W.LARGE_P, W.LARGE_N
     marks = sorted(k[1] for k in gmp_fixtures if k[0] == p)
     assert marks, "missing GMP fixture for p=%d" % p
-    announce("C2 p=%d n=%d marks=%s schedule=%s" % (p, n, marks, schedule))
+    announce("77M at 4M words p=%d n=%d marks=%s schedule=%s" % (p, n, marks, schedule))
     kw = {} if schedule is None else dict(width=schedule[0], height=schedule[1], lines=schedule[2],
                                            flags=L.LL_OPT_FORCE_SWEEPS)
     with L.LLContext(p, n, **kw) as c:
@@ -204,22 +204,22 @@ def test_c2_77m_vs_gmp_fixture(gmp_fixtures, schedule):
             fx = gmp_fixtures[(p, mk)]
             assert "0x%016x" % r == fx["res64"], "iteration %d" % mk
             assert sha(w) == fx["sha256"], "iteration %d" % mk
-    print("C2 max roundoff over %d iterations: %.4f" % (done, worst))
+    print("77M at 4M words: max roundoff over %d iterations: %.4f" % (done, worst))
     assert worst < ROUNDOFF_LIMIT
 
 
-@pytest.mark.parametrize("n", W.C3_LENGTHS)
-def test_c3_smooth_lengths_vs_gmp_fixture(n, gmp_fixtures):
-    p = W.c3_prime(n)
+@pytest.mark.parametrize("n", W.SMOOTH_LENGTHS)
+def test_smooth_lengths_vs_gmp_fixture(n, gmp_fixtures):
+    p = W.smooth_prime(n)
     fx = gmp_fixtures[(p, 200)]
-    announce("C3 p=%d n=%d" % (p, n))
+    announce("smooth length p=%d n=%d" % (p, n))
     with L.LLContext(p, n) as c:
         c.load_initial(0)
         m = c.iterate(200)
         w, r = c.residue()
     assert "0x%016x" % r == fx["res64"]
     assert sha(w) == fx["sha256"]
-    print("C3 n=%d p=%d bits/word=%.3f max roundoff %.4f" % (n, p, p / n, m))
+    print("smooth length n=%d p=%d bits/word=%.3f max roundoff %.4f" % (n, p, p / n, m))
     assert m < 0.45
 
 

@@ -158,7 +158,7 @@ def test_selector_saves_time_on_seeded_exponents_with_exact_residues():
     batched time against the reference policy, every residue after the
     round-off test equals the reference-length run's, and Res64 equals GMP
     for the exponents below 2e6."""
-    primes = W.log_uniform_primes(W.C5_EVAL_SEED, W.C5_EVAL_COUNT, *W.C5_EVAL_RANGE)[::8]
+    primes = W.log_uniform_primes(W.SELECTOR_EVAL_SEED, W.SELECTOR_EVAL_COUNT, *W.SELECTOR_EVAL_RANGE)[::8]
     times = S.read_times(S.TIMES_PATH)
     announce("selector evaluation on %d exponents" % len(primes))
     from cudalucas_amd import tuner as T
@@ -193,7 +193,7 @@ def test_selector_full_evaluation_all_residues_vs_gmp(tmp_path):
 
     from cudalucas_amd import tuner as T
 
-    primes = W.log_uniform_primes(W.C5_EVAL_SEED, W.C5_EVAL_COUNT, *W.C5_EVAL_RANGE)
+    primes = W.log_uniform_primes(W.SELECTOR_EVAL_SEED, W.SELECTOR_EVAL_COUNT, *W.SELECTOR_EVAL_RANGE)
     times = S.read_times(S.TIMES_PATH)
     btimes = S.read_times(S.BATCH_TIMES_PATH)
     sched = T.load(T.BATCH_TABLE)

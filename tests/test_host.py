@@ -179,7 +179,7 @@ def test_fft_from_str():
 def test_supported_lengths_cover_the_configs():
     from cudalucas_amd import workloads as W
 
-    for n in [W.C1_N, W.C2_N] + W.C3_LENGTHS:
+    for n in [W.SMALL_N, W.LARGE_N] + W.SMOOTH_LENGTHS:
         assert L.fft_supported(n)
 
 

@@ -60,10 +60,10 @@ def test_gmp_composite_mersenne_nonzero(p):
 def test_seeded_primes_agree_with_product_definition():
     from cudalucas_amd import workloads as W
 
-    assert O.seeded_prime(W.C1_SEED, 40000, 50000) == W.C1_P
-    assert O.seeded_prime(W.C2_SEED, 77_000_000, 77_100_000) == W.C2_P
-    for n in W.C3_LENGTHS:
-        assert O.is_prime(W.c3_prime(n))
+    assert O.seeded_prime(W.SMALL_SEED, 40000, 50000) == W.SMALL_P
+    assert O.seeded_prime(W.LARGE_SEED, 77_000_000, 77_100_000) == W.LARGE_P
+    for n in W.SMOOTH_LENGTHS:
+        assert O.is_prime(W.smooth_prime(n))
 
 
 @pytest.mark.parametrize("m", [1, 2, 3, 4, 5, 6, 7, 8, 12, 14, 15, 16, 21, 35, 49, 64, 96, 105, 128])
