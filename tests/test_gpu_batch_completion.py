@@ -8,7 +8,7 @@ Mersenne-prime exponent ends at residue 0 and every other one does not;
 no iteration of any test exceeds the 0.35 roundoff limit (so there is no
 undetected failure to look for); and for the smallest exponents the full
 final residue equals the plain single-exponent engine's, run to completion
-beside it.  The run takes about 40 GPU-minutes, so it is split in four
+beside it.  The run takes about half an hour of GPU time, so it is split in four
 parts and only runs when asked:
 
     LL_BATCH_COMPLETION=low  python -m pytest -s tests/test_gpu_batch_completion.py   # p < 2e6
@@ -42,7 +42,7 @@ def digest(words):
     return hashlib.sha256(np.ascontiguousarray(words, dtype="<u4").tobytes()).hexdigest()[:32]
 
 
-@pytest.mark.skipif(PART not in PARTS, reason="set LL_BATCH_COMPLETION=low|mid|high|top (8-12 GPU-minutes each)")
+@pytest.mark.skipif(PART not in PARTS, reason="set LL_BATCH_COMPLETION=low|mid|high|top (4-10 GPU-minutes each)")
 def test_batch_to_completion():
     ps = W.log_uniform_primes(W.BATCH_SEED, W.BATCH_COUNT, *W.BATCH_RANGE)
     ps = sorted(set(ps) | set(MERSENNE_PRIME_EXPONENTS))
