@@ -464,7 +464,10 @@ __global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArg
   // below b, q above; the factors of 2 are exact ldexp scalings.
   const int q = (int)A.q;
   double emax = 0.0;
-  auto round_column = [&](const int a, const int fa, const double iwa) {
+  // 512-thread workgroups keep the balanced digits of their columns in
+  // registers until the carry-in arrives; the others park them in LDS
+  double dg[PF ? CPT : 1][PF ? 2 * R : 1];
+  auto round_column = [&](const int a, const int c, const int fa, const double iwa) {
     // m2i < 2R: this chain takes the -2, at word m2i (one chain of all)
     const uint32_t m2i = m2_digit - 2u * ((uint32_t)a * H + b0);
     double carry = 0.0;
@@ -485,7 +488,12 @@ __global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArg
           if constexpr (decltype(with_m2)::value) rt -= (uint32_t)i == m2i ? dmod : 0.0;
           d[e] = balance_f64(rt + carry, q + big, carry);
         }
-        lds[r * LP + lpad(a)] = {d[0], d[1]};
+        if constexpr (PF) {
+          dg[c][2 * r] = d[0];
+          dg[c][2 * r + 1] = d[1];
+        } else {
+          lds[r * LP + lpad(a)] = {d[0], d[1]};
+        }
       }
     };
     if (m2i < 2u * R) chain(std::true_type{});
@@ -500,14 +508,14 @@ __global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArg
   if constexpr (PF) {
 #pragma unroll
     for (int c = 0; c < CPT; c++)
-      if (W % T == 0 || tid + c * T < W) round_column(tid + c * T, fa_c[c], iwa_c[c]);
+      if (W % T == 0 || tid + c * T < W) round_column(tid + c * T, c, fa_c[c], iwa_c[c]);
   } else {
-    for (int a = tid; a < W; a += T) round_column(a, gload(A.fA + a), gload(A.iwA + a));
+    for (int a = tid; a < W; a += T) round_column(a, 0, gload(A.fA + a), gload(A.iwA + a));
   }
 
   // carry-in from the previous block (block 0: the last block, one column
   // to the left), propagated through the whole chain, then reweight
-  auto carry_in_column = [&](const int a, const int fa, const double wa) {
+  auto carry_in_column = [&](const int a, const int c, const int fa, const double wa) {
     const size_t src = blk == 0 ? (size_t)(G - 1) * W + (a + W - 1) % W : (size_t)(blk - 1) * W + a;
     uint64_t word;
     uint32_t spins = 0;
@@ -528,7 +536,9 @@ __global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArg
     double cc = (double)(int32_t)(uint32_t)word;
 #pragma unroll
     for (int r = 0; r < R; r++) {
-      const cplx dv = lds[r * LP + lpad(a)];
+      cplx dv;
+      if constexpr (PF) dv = {dg[c][2 * r], dg[c][2 * r + 1]};
+      else dv = lds[r * LP + lpad(a)];
       double x[2];
 #pragma unroll
       for (int e = 0; e < 2; e++) {
@@ -546,9 +556,9 @@ __global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArg
   if constexpr (PF) {
 #pragma unroll
     for (int c = 0; c < CPT; c++)
-      if (W % T == 0 || tid + c * T < W) carry_in_column(tid + c * T, fa_c[c], wa_c[c]);
+      if (W % T == 0 || tid + c * T < W) carry_in_column(tid + c * T, c, fa_c[c], wa_c[c]);
   } else {
-    for (int a = tid; a < W; a += T) carry_in_column(a, gload(A.fA + a), gload(A.wA + a));
+    for (int a = tid; a < W; a += T) carry_in_column(a, 0, gload(A.fA + a), gload(A.wA + a));
   }
   __syncthreads();
   fft_lds_auto<W, R, T, false, TAB>(lds, tw, ptab);
