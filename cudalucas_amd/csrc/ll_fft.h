@@ -32,16 +32,28 @@ template <typename V>
 __device__ __forceinline__ V gload(const V *p) {
   return *(const __attribute__((address_space(1))) V *)p;
 }
-// Streaming (non-temporal) forms for the signal, which each sweep reads and
-// writes exactly once.
+// Streaming forms for the signal, which each sweep reads and writes exactly
+// once: non-temporal loads, and stores written through to memory (cache
+// policy sc0 sc1: a 32 MiB store phase takes about 8.0 us instead of 9.2
+// with the default or the non-temporal policy, measured on MI355X).  The
+// stores go through a buffer resource over the whole signal (raw buffer
+// stores are the form whose cache policy the compiler takes as an
+// argument); word 3 of the resource is the gfx9 untyped-buffer value.
 __device__ __forceinline__ cplx gload_stream(const cplx *p) {
   const gd2 v = __builtin_nontemporal_load((const __attribute__((address_space(1))) gd2 *)p);
   return {v.x, v.y};
 }
-__device__ __forceinline__ void gstore_stream(cplx *p, cplx c) {
-  const gd2 v = {c.x, c.y};
-  __builtin_nontemporal_store(v, (__attribute__((address_space(1))) gd2 *)p);
-}
+struct StreamOut {
+  __amdgpu_buffer_rsrc_t rsrc;
+  __device__ __forceinline__ StreamOut(cplx *base, uint32_t count)
+      : rsrc(__builtin_amdgcn_make_buffer_rsrc(base, 0, (int)(count * sizeof(cplx)), 0x00020000)) {}
+  // element `index` of the signal <- c
+  __device__ __forceinline__ void store(uint32_t index, cplx c) const {
+    typedef unsigned u4 __attribute__((ext_vector_type(4)));
+    const u4 v = __builtin_bit_cast(u4, gd2{c.x, c.y});
+    __builtin_amdgcn_raw_buffer_store_b128(v, rsrc, (int)(index * sizeof(cplx)), 0, 1 | 16);  // sc0 | sc1
+  }
+};
 
 __device__ __forceinline__ cplx cadd(cplx a, cplx b) { return {a.x + b.x, a.y + b.y}; }
 __device__ __forceinline__ cplx csub(cplx a, cplx b) { return {a.x - b.x, a.y - b.y}; }

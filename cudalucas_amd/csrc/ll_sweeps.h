@@ -299,8 +299,6 @@ __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict_
     }
   }
   __syncthreads();
-  cplx *out0 = A.x + ((size_t)c0 << rs);
-  cplx *out1 = A.x + ((size_t)c1 << rs);
   if constexpr (FUSED) {
     constexpr int PER = 2 * QL / T;
     // the last pass's row twiddles w^(c j), loaded while the other passes run
@@ -325,30 +323,32 @@ __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict_
         v[s][i] = a;
       }
     }
+    const StreamOut out(A.x, A.n / 2);
 #pragma unroll
     for (int s = 0; s < PER; s++) {
       const int u = tid + s * T, b = u / QL, j = u % QL;
-      cplx *col = b ? out1 : out0;
+      const uint32_t col = cols[b] << rs;
       dft<RL, true>(v[s]);
       const cplx tj = tjL[s];
 #pragma unroll
       for (int i = 0; i < RL; i++) {
         const int row = j + i * QL;
         const cplx w = i ? cmul(tj, colfL[b * RL + i]) : tj;
-        gstore_stream(col + (((size_t)(row >> rs) * W << rs) | (row & rmask)), cmulc(v[s][i], w));
+        out.store(col + ((((uint32_t)row >> rs) * W << rs) | (row & rmask)), cmulc(v[s][i], w));
       }
     }
   } else {
     fft_lds_cached<H, 2, T, true>(lds, ptw);
     const cplx t0 = twiddle_m(A.twM, c0 * (uint32_t)tid), t1 = twiddle_m(A.twM, c1 * (uint32_t)tid);
+    const StreamOut out(A.x, A.n / 2);
 #pragma unroll 2
     for (int st = 0; st < H / T; st++) {
       const int i = tid + st * T;
-      const size_t off = ((size_t)(i >> rs) * W << rs) | (i & rmask);
+      const uint32_t off = (((uint32_t)i >> rs) * W << rs) | (i & rmask);
       const cplx w0 = cmul(t0, twiddle_m(A.twM, c0 * (uint32_t)(st * T)));
       const cplx w1 = cmul(t1, twiddle_m(A.twM, c1 * (uint32_t)(st * T)));
-      gstore_stream(out0 + off, cmulc(lds[lpad(i)], w0));
-      gstore_stream(out1 + off, cmulc(lds[LP + lpad(i)], w1));
+      out.store((c0 << rs) + off, cmulc(lds[lpad(i)], w0));
+      out.store((c1 << rs) + off, cmulc(lds[LP + lpad(i)], w1));
     }
   }
 }
@@ -574,10 +574,11 @@ __global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArg
       const int e = t2 + s * T;
       if ((R * W) % T == 0 || e < R * W) v[s] = lds[(e % R) * LP + lpad(e / R)];
     }
+    const StreamOut out(A.x, A.n / 2);
 #pragma unroll
     for (int s = 0; s < NE; s++) {
       const int e = tid + s * T;
-      if ((R * W) % T == 0 || e < R * W) gstore_stream(A.x + (size_t)b0 * W + e, v[s]);
+      if ((R * W) % T == 0 || e < R * W) out.store(b0 * W + e, v[s]);
     }
   }
   const float m = block_max<T>((float)emax, red);
