@@ -467,10 +467,14 @@ __global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArg
   // 512-thread workgroups keep the balanced digits of their columns in
   // registers until the carry-in arrives; the others park them in LDS
   double dg[PF ? CPT : 1][PF ? 2 * R : 1];
+  // and the wrap and big flags of each word (bits 2i and 2i + 1)
+  static_assert(2 * R <= 32, "two flag bits per word of the chain");
+  uint64_t flags[PF ? CPT : 1];
   auto round_column = [&](const int a, const int c, const int fa, const double iwa) {
     // m2i < 2R: this chain takes the -2, at word m2i (one chain of all)
     const uint32_t m2i = m2_digit - 2u * ((uint32_t)a * H + b0);
     double carry = 0.0;
+    uint64_t fl = 0;
     auto chain = [&](auto with_m2) {
 #pragma unroll
       for (int r = 0; r < R; r++) {
@@ -482,6 +486,7 @@ __global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArg
           const int f0 = fa + fb[i];
           const int wrap = f0 >= (int)A.n;
           const int big = f0 - (wrap ? (int)A.n : 0) < (int)A.b;
+          if constexpr (PF) fl |= ((uint64_t)wrap | ((uint64_t)big << 1)) << (2 * i);
           const double t = ldexp((e ? v.y : v.x) * (iwa * iwb[i]), wrap);
           double rt = rint(t);
           emax = fmax(emax, fabs(t - rt));
@@ -498,6 +503,7 @@ __global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArg
     };
     if (m2i < 2u * R) chain(std::true_type{});
     else chain(std::false_type{});
+    if constexpr (PF) flags[c] = fl;
     const bool fits = fabs(carry) < 2147483648.0;
     if (!fits) emax = 2.0;  // carry overflow: flagged as a failed iteration
     const int32_t c32 = fits ? (int32_t)carry : 0;
@@ -543,9 +549,15 @@ __global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArg
 #pragma unroll
       for (int e = 0; e < 2; e++) {
         const int i = 2 * r + e;
-        const int f0 = fa + fb[i];
-        const int wrap = f0 >= (int)A.n;
-        const int width = q + (f0 - (wrap ? (int)A.n : 0) < (int)A.b);
+        int wrap, width;
+        if constexpr (PF) {
+          wrap = (int)(flags[c] >> (2 * i)) & 1;
+          width = q + ((int)(flags[c] >> (2 * i + 1)) & 1);
+        } else {
+          const int f0 = fa + fb[i];
+          wrap = f0 >= (int)A.n;
+          width = q + (f0 - (wrap ? (int)A.n : 0) < (int)A.b);
+        }
         double d = balance_f64((e ? dv.y : dv.x) + cc, width, cc);
         if (i == 2 * R - 1) d += ldexp(cc, width);
         x[e] = ldexp(d * (wa * wb[i]), -wrap);
