@@ -5,7 +5,7 @@
 // The reference picks lengths from fixed power-law bounds
 // (choose_fft_length, CUDALucas.cu:1027-1068) and only learns that a length
 // was too short from the round-off test (CUDALucas.cu:2589-2648); the
-// selector keeps that test as its fallback judge (ll_selector_choose with
+// selector keeps that test as its fallback check (ll_selector_choose with
 // `above` set returns the next candidate).
 #include <math.h>
 #include <stdarg.h>

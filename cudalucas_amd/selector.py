@@ -7,7 +7,7 @@ any of the first 1000 iterations above 0.35, or their average above 0.25,
 restarts at the next length).  Here an FP32 MLP (hand-written HIP kernels,
 ll_mlp_*) predicts the maximum roundoff of (p, n) from seven features, and
 ll_selector_choose takes the fastest length predicted to stay under 0.35 over
-a whole test; the round-off test stays as the fallback judge.
+a whole test; the round-off test stays as the fallback check.
 
 Pipeline (every GPU step goes through libcudalucas_amd.so):
 
@@ -62,7 +62,7 @@ WINDOWS = 8          # windows per trace
 WINDOW_ITERS = 16    # iterations per window
 LIMIT = 0.35         # careful round-off test: per-iteration maximum
 LIMIT_AVG = 0.25     # careful round-off test: average over the test
-JUDGE_ITERS = 1000   # careful round-off test length
+CHECK_ITERS = 1000   # careful round-off test length
 CLIP_LO = 2.0 ** -24
 
 
@@ -375,7 +375,7 @@ def _digest(words):
     return hashlib.sha256(np.ascontiguousarray(words, dtype="<u4").tobytes()).hexdigest()[:32]
 
 
-def _run_judged(tests, iters, words_budget=1 << 27, max_tests=256):
+def _run_checked(tests, iters, words_budget=1 << 27, max_tests=256):
     """[(max, mean, res64, digest)] per (p, n, offset) after `iters`
     iterations in the batched engine (per-iteration errors kept)."""
     out = [None] * len(tests)
@@ -429,8 +429,8 @@ def reference_choices(primes, lengths=None):
             L.ll_fft_length_list_set(0, None)
 
 
-def _select_and_judge(selector, primes, offs, times, judge_iters, log=None, tag=""):
-    """First choices, the round-off judge, fallbacks until every exponent
+def _select_and_check(selector, primes, offs, times, check_iters, log=None, tag=""):
+    """First choices, the round-off check, fallbacks until every exponent
     has an accepted length.  Returns (first, accepted, fallbacks, results,
     wasted ms) with results[i] = (max err, mean err, res64, digest)."""
     cur = [selector.choose(p)[0] for p in primes]
@@ -443,7 +443,7 @@ def _select_and_judge(selector, primes, offs, times, judge_iters, log=None, tag=
     rounds = 0
     while pending and rounds < 8:
         rounds += 1
-        res = _run_judged([(primes[i], cur[i], offs[i]) for i in pending], judge_iters)
+        res = _run_checked([(primes[i], cur[i], offs[i]) for i in pending], check_iters)
         nxt = []
         for i, (e, r, dg) in zip(pending, res):
             if careful_test(e):
@@ -451,7 +451,7 @@ def _select_and_judge(selector, primes, offs, times, judge_iters, log=None, tag=
                 result[i] = (float(e.max()), float(e.mean()), r, dg)
             else:
                 fallbacks[i] += 1
-                wasted_ms += judge_iters * times[cur[i]]
+                wasted_ms += check_iters * times[cur[i]]
                 cur[i] = selector.choose(primes[i], above=cur[i])[0]
                 nxt.append(i)
         pending = nxt
@@ -476,11 +476,11 @@ def _batch_iteration_ms(tests, iters, words_budget=1 << 27, max_tests=256):
     return total
 
 
-def evaluate(model_path, primes, times, batch_times=None, schedules=None, judge_iters=JUDGE_ITERS,
+def evaluate(model_path, primes, times, batch_times=None, schedules=None, check_iters=CHECK_ITERS,
              batch_iters=512, seed=W.SELECTOR_EVAL_SEED, log=None):
     """The learned selector against the reference policy on `primes`.
 
-    Two uses of the same model, each through the round-off judge with
+    Two uses of the same model, each through the round-off check with
     fallback, each with residues compared with the reference-length run:
 
     * "timed": the fastest length predicted safe by the plain engine's
@@ -502,10 +502,10 @@ def evaluate(model_path, primes, times, batch_times=None, schedules=None, judge_
         tuner.install(schedules)
     try:
         with Selector(model_path, times=times) as st:
-            t_first, t_acc, t_fb, t_res, t_waste = _select_and_judge(st, primes, offs, times, judge_iters, log, "timed")
+            t_first, t_acc, t_fb, t_res, t_waste = _select_and_check(st, primes, offs, times, check_iters, log, "timed")
         with Selector(model_path, times=batch_times) as sb:
-            b_first, b_acc, b_fb, b_res, _ = _select_and_judge(sb, primes, offs, times, judge_iters, log, "batched")
-        ref = _run_judged([(p, n, o) for p, n, o in zip(primes, nref, offs)], judge_iters)
+            b_first, b_acc, b_fb, b_res, _ = _select_and_check(sb, primes, offs, times, check_iters, log, "batched")
+        ref = _run_checked([(p, n, o) for p, n, o in zip(primes, nref, offs)], check_iters)
         b_ref = _batch_iteration_ms([(p, n, o) for p, n, o in zip(primes, nref, offs)], batch_iters)
         b_sel = _batch_iteration_ms([(p, n, o) for p, n, o in zip(primes, b_acc, offs)], batch_iters)
     finally:
@@ -527,8 +527,8 @@ def evaluate(model_path, primes, times, batch_times=None, schedules=None, judge_
     return {
         "exponents": len(primes),
         "range": [min(primes), max(primes)],
-        "judge": "careful round-off test: %d iterations, max <= %.2f and mean <= %.2f" % (judge_iters, LIMIT, LIMIT_AVG),
-        "reference_length_judge_failures": ref_fail,
+        "check": "careful round-off test: %d iterations, max <= %.2f and mean <= %.2f" % (check_iters, LIMIT, LIMIT_AVG),
+        "reference_length_check_failures": ref_fail,
         "timed": {
             "shorter_than_reference": sum(a < b for a, b in zip(t_acc, nref)),
             "longer_than_reference": sum(a > b for a, b in zip(t_acc, nref)),

@@ -83,7 +83,7 @@ def test_batch_forced_sweeps_small_lengths():
 
 def test_large_multi_group_batch_has_no_failed_iterations():
     """A batch of several large two-sweep groups (256K-1M words, 8 tests
-    each, the shape of the round-off judge's batches): every iteration's
+    each, the shape of the round-off check's batches): every iteration's
     error stays far below the limit, so no carry wait timed out (those
     report 3.0), and residues equal the plain engine's.  Regression test
     for carry launches of different groups sharing the GPU."""

@@ -170,14 +170,14 @@ def test_selector_saves_time_on_seeded_exponents_with_exact_residues():
     for form in ("timed", "batched"):
         assert rep[form]["residue_mismatches_vs_reference_length"] == []
         assert rep[form]["max_roundoff"] <= S.LIMIT
-    assert rep["reference_length_judge_failures"] == 0
+    assert rep["reference_length_check_failures"] == 0
     assert rep["time_saved_fraction_plain"] > 0.0
     # the batched engine at the selected lengths against the plain
     # per-exponent path at the reference policy's lengths
     assert rep["batched_selected_vs_plain_reference"] > 2.0
     for row in rep["per_exponent"]:
         if row["p"] < 2_000_000:
-            g = O.gmp_ll(row["p"], S.JUDGE_ITERS)
+            g = O.gmp_ll(row["p"], S.CHECK_ITERS)
             assert "%016x" % O.res64(g) == row["res64"], row
 
 
@@ -206,7 +206,7 @@ def test_selector_full_evaluation_all_residues_vs_gmp(tmp_path):
     for form in ("timed", "batched"):
         assert rep[form]["residue_mismatches_vs_reference_length"] == []
     with Pool(16) as pool:
-        want = pool.starmap(_gmp_res64, [(p, S.JUDGE_ITERS) for p in primes])
+        want = pool.starmap(_gmp_res64, [(p, S.CHECK_ITERS) for p in primes])
     got = {row["p"]: row["res64"] for row in rep["per_exponent"]}
     bad = [p for p, w in zip(primes, want) if got[p] != w]
     print("GMP Res64 checked for %d exponents, %d mismatches" % (len(primes), len(bad)))

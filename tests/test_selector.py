@@ -5,7 +5,7 @@ restating it: central finite differences of the loss, the closed-form
 least-squares gradient of a single linear layer, tanh's derivative through a
 one-hidden-unit network written out by hand, and Adam's first step, which is
 lr * sign(g) up to eps.  Host-side selector pieces (features, candidate
-lengths, model-file grammar, the round-off judge) run through the C-ABI
+lengths, model-file grammar, the round-off check) run through the C-ABI
 library without a GPU.
 """
 import math
