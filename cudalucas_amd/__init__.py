@@ -162,6 +162,11 @@ def lib():
     L.ll_selector_set_times.argtypes = [P, u32, P, P]
     L.ll_selector_predict.argtypes = [P, u32, u32, P, P]
     L.ll_selector_choose.argtypes = [P, u32, u32, ctypes.POINTER(f32)]
+    L.ll_work_parse_line.argtypes = [ctypes.c_char_p, ctypes.POINTER(Assignment)]
+    L.ll_is_prime.argtypes = [u32]
+    L.ll_valid_assignment.argtypes = [u32, u32]
+    L.ll_work_next.argtypes = [ctypes.c_char_p, ctypes.POINTER(Assignment)]
+    L.ll_work_clear.argtypes = [ctypes.c_char_p, u32]
     _lib = L
     return L
 
@@ -263,6 +268,38 @@ def fft_schedule(n, width=0, height=0, lines=0, flags=0):
 
 def fft_from_str(s):
     return lib().ll_fft_from_str(s.encode())
+
+
+# ---------------------------------------------------------------- worktodo
+
+WORK_OK, WORK_EOF, WORK_LONG_LINE, WORK_NO_TEST_EQUAL = 0, 1, 2, 3
+WORK_INVALID_FORMAT, WORK_INVALID_DATA, WORK_BLANK, WORK_COMMENT = 4, 5, 6, 7
+
+
+class Assignment(ctypes.Structure):
+    """ll_assignment: one parsed work line (struct ASSIGNMENT, parse.c:230-236)."""
+    _fields_ = [("exponent", ctypes.c_uint32), ("fft_length", ctypes.c_uint32),
+                ("key", ctypes.c_char * 192), ("comment", ctypes.c_char * 192)]
+
+
+def work_parse_line(line):
+    """parse_worktodo_line (parse.c:241-484): (status, Assignment)."""
+    a = Assignment()
+    st = lib().ll_work_parse_line(line.encode(), ctypes.byref(a))
+    return st, a
+
+
+def work_next(path):
+    """get_next_assignment (parse.c:486-554): (0, Assignment), or (1 = cannot
+    open, 2 = no valid line, None)."""
+    a = Assignment()
+    rc = lib().ll_work_next(os.fsencode(path), ctypes.byref(a))
+    return rc, (a if rc == 0 else None)
+
+
+def work_clear(path, p):
+    """clear_assignment (parse.c:556-678): 0, or 3/4/5/6 as the reference."""
+    return lib().ll_work_clear(os.fsencode(path), p)
 
 
 # ----------------------------------------------------------------- engine

@@ -596,61 +596,6 @@ static int check_residue(const Config &cfg, int longtest) {
   return bad ? 1 : 0;
 }
 
-// ------------------------------------------------------------- worktodo
-// parse_worktodo_line (parse.c:241-484): a "Test=" or "DoubleCheck=" line;
-// an optional assignment key (32 hex digits or N/A) precedes the exponent.
-static bool parse_work_line(const char *line, uint32_t &q, std::string &aid) {
-  const char *eq = nullptr;
-  if (!strncasecmp(line, "Test=", 5)) eq = line + 5;
-  else if (!strncasecmp(line, "DoubleCheck=", 12)) eq = line + 12;
-  if (!eq) return false;
-  std::vector<std::string> fields(1);
-  for (const char *c = eq; *c && *c != '\n' && *c != '\r'; c++) {
-    if (*c == ',') fields.emplace_back();
-    else if (*c != ' ') fields.back() += *c;
-  }
-  size_t k = 0;
-  aid.clear();
-  if (fields[0].size() == 32 || !strcasecmp(fields[0].c_str(), "N/A")) aid = fields[k++];
-  if (k >= fields.size()) return false;
-  q = (uint32_t)strtoul(fields[k].c_str(), nullptr, 10);
-  return q > 0;
-}
-
-// get_next_assignment (parse.c:486-571): the first valid work line.
-static bool next_assignment(const std::string &file, uint32_t &q, std::string &aid) {
-  FILE *f = fopen(file.c_str(), "r");
-  if (!f) return false;
-  char line[1024];
-  bool found = false;
-  while (!found && fgets(line, sizeof line, f)) found = parse_work_line(line, q, aid);
-  fclose(f);
-  return found;
-}
-
-// clear_assignment (parse.c:572-678): drop the finished line.
-static void clear_assignment(const std::string &file, uint32_t q) {
-  FILE *f = fopen(file.c_str(), "r");
-  if (!f) return;
-  std::vector<std::string> keep;
-  char line[1024];
-  bool dropped = false;
-  while (fgets(line, sizeof line, f)) {
-    uint32_t lq = 0;
-    std::string aid;
-    if (!dropped && parse_work_line(line, lq, aid) && lq == q) {
-      dropped = true;
-      continue;
-    }
-    keep.push_back(line);
-  }
-  fclose(f);
-  f = fopen(file.c_str(), "w");
-  if (!f) return;
-  for (auto &l : keep) fputs(l.c_str(), f);
-  fclose(f);
-}
-
 // ------------------------------------------------------- tuning files
 // "<device name> fft.txt" and "<device name> schedules.txt" in the working
 // directory, as the reference keeps "<device name> fft.txt" and
@@ -1052,21 +997,32 @@ int main(int argc, char **argv) {
   }
   if (cfg.selftest >= 0) return check_residue(cfg, cfg.selftest);
   if (q) {
-    // valid_assignment (parse.c:176-203)
-    if (q < 7500) { fprintf(stderr, "Warning: exponents < 7500 are not supported!\n"); return 2; }
-    if (cfg.fftlen % 1024) { fprintf(stderr, "Warning: FFT length %d is invalid, it must be a multiple of 1024.\n", cfg.fftlen); return 2; }
+    // valid_assignment; a composite exponent moves on to the next prime
+    // (CUDALucas.cu:3560-3575)
+    const int valid = ll_valid_assignment(q, cfg.fftlen > 0 ? (uint32_t)cfg.fftlen : 0);
+    if (valid == 0) return 2;
+    if (valid < 0) {
+      uint32_t next = q | 1;
+      while (next <= 0x7fffffffu && !ll_is_prime(next)) next += 2;
+      if (next > 0x7fffffffu) {
+        fprintf(stderr, "Warning: exponent is too big besides!\n");
+        return 2;
+      }
+      fprintf(stderr, "Using %u instead.\n", next);
+      q = next;
+    }
     return check(q, cfg, nullptr);
   }
+  // The worktodo loop (CUDALucas.cu:3549-3558, parse.c:486-678): a length
+  // given on the work line holds for that test only.
   for (;;) {
-    std::string aid;
-    uint32_t wq = 0;
-    if (!next_assignment(cfg.workfile, wq, aid)) {
-      fprintf(stderr, "No valid assignment found in %s\n", cfg.workfile.c_str());
-      return 2;
-    }
-    const int rc = check(wq, cfg, aid.c_str());
+    ll_assignment work;
+    if (ll_work_next(cfg.workfile.c_str(), &work)) return 2;
+    Config this_test = cfg;
+    if (work.fft_length) this_test.fftlen = (int)work.fft_length;
+    const int rc = check(work.exponent, this_test, work.key);
     if (rc) return rc;
     if (g_quit) return 0;
-    clear_assignment(cfg.workfile, wq);
+    if (ll_work_clear(cfg.workfile.c_str(), work.exponent)) return 2;
   }
 }

@@ -1,13 +1,17 @@
 // Host-side arithmetic of the LL engine: variable-base digit codec, Res64,
-// checkpoint checksum, FFT-length policy and the extended-precision tables
-// (roots of unity, IBDWT weights) the kernels read.
+// checkpoint checksum, FFT-length policy, the worktodo file and the
+// extended-precision tables (roots of unity, IBDWT weights) the kernels read.
+#include <ctype.h>
+#include <errno.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <strings.h>
 #include <sys/stat.h>
 #include <sys/types.h>
 
+#include <string>
 #include <vector>
 
 #include "ll_api.h"
@@ -423,6 +427,237 @@ int ll_fft_from_str(const char *s) {
   const unsigned long long n = (unsigned long long)v * mult;
   if (n > 0x7fffffffull) return -1;
   return (int)n;
+}
+
+// ---------------------------------------------------------------- worktodo
+
+// A work line holds at most 190 characters before its newline: the
+// reference reads lines into a 192-byte buffer and rejects a line that fills
+// it (MAX_LINE_LENGTH, parse.c:67, 280-283).
+static const size_t kWorkLineMax = 190;
+
+static void copy_text(char *dst, const char *b, const char *e) {
+  while (e > b && isspace((unsigned char)e[-1])) e--;
+  size_t k = (size_t)(e - b);
+  if (k >= LL_WORK_TEXT) k = LL_WORK_TEXT - 1;
+  memcpy(dst, b, k);
+  dst[k] = 0;
+}
+
+// A number field: digits from b, then nothing but an optional unit letter at
+// u (K or M, u == nullptr for a plain number) and ignored trailing text.
+// Returns LL_WORK_OK and the value, or the status that rejects the field
+// (parse.c:372-430).
+static int work_number(const char *b, const char *u, unsigned long long scale, unsigned long long &v) {
+  char *stop = nullptr;
+  errno = 0;
+  const unsigned long x = strtoul(b, &stop, 10);
+  if (stop == b) return LL_WORK_INVALID_FORMAT;
+  if (u && stop < u) return LL_WORK_INVALID_FORMAT;  // "40 96K", "4096 K"
+  if (errno || x > 0x7fffffffull / scale) return LL_WORK_INVALID_DATA;
+  v = (unsigned long long)x * scale;
+  return LL_WORK_OK;
+}
+
+int ll_work_parse_line(const char *line, ll_assignment *a) {
+  if (!a) return LL_ERR_ARG;
+  memset(a, 0, sizeof *a);
+  if (!line) return LL_WORK_EOF;
+  const size_t len = strcspn(line, "\n");
+  if (len > kWorkLineMax) return LL_WORK_LONG_LINE;
+  char text[kWorkLineMax + 1];  // the line alone, so no scan runs past it
+  memcpy(text, line, len);
+  text[len] = 0;
+  const char *end = text + len;
+  const char *s = text;
+  while (s < end && isspace((unsigned char)*s)) s++;
+  if (s == end) return LL_WORK_BLANK;
+  if (*s == '#' || (end - s >= 2 && (s[0] == '/' || s[0] == '\\') && s[1] == s[0])) return LL_WORK_COMMENT;
+  if (strncasecmp(s, "Test=", 5) && strncasecmp(s, "DoubleCheck=", 12)) return LL_WORK_NO_TEST_EQUAL;
+  s = strchr(s, '=') + 1;
+  // The assignment ends at a comment delimiter (#, // or \\).
+  const char *stop = end, *note = nullptr;
+  for (const char *c = s; c < end && !note; c++) {
+    if (*c == '#') stop = c, note = c + 1;
+    else if (c + 1 < end && (*c == '/' || *c == '\\') && c[1] == *c) stop = c, note = c + 2;
+  }
+  int commas = 0;
+  for (const char *c = s; c < stop; c++) commas += *c == ',';
+  if (commas > 4) return LL_WORK_INVALID_FORMAT;  // key, exponent, length, TF bits, P-1 done
+  // Fields (parse.c:344-437): the first letter in a field decides its kind;
+  // a-f marks the assignment key (hex digits or N/A), K or M after digits an
+  // FFT length; a field without one is a number, and the largest number is
+  // the exponent (trial-factoring depth and the P-1 flag are smaller).
+  for (const char *f = s; f <= stop;) {
+    while (f < stop && isspace((unsigned char)*f)) f++;
+    const char *fe = f;
+    while (fe < stop && *fe != ',') fe++;
+    const char *kind = f;
+    while (kind < fe && !strchr("abcdefABCDEFkKmM", *kind)) kind++;
+    // A leading M is a length without digits, so "M<p>" is rejected as in
+    // the reference (its M<p> exponent branch, parse.c:418-419, is never
+    // reached).
+    int st = LL_WORK_OK;
+    unsigned long long v = 0;
+    if (kind < fe && (*kind == 'k' || *kind == 'K')) {
+      st = work_number(f, kind, 1024, v);
+      a->fft_length = (uint32_t)v;
+    } else if (kind < fe && (*kind == 'm' || *kind == 'M')) {
+      st = work_number(f, kind, 1024 * 1024, v);
+      a->fft_length = (uint32_t)v;
+    } else if (kind < fe) {
+      copy_text(a->key, f, fe);
+    } else {
+      st = work_number(f, nullptr, 1, v);
+      if (st == LL_WORK_OK && v > a->exponent) a->exponent = (uint32_t)v;
+    }
+    if (st != LL_WORK_OK) return st;
+    f = fe + 1;
+  }
+  if (note) {
+    while (note < end && isspace((unsigned char)*note)) note++;
+    copy_text(a->comment, note, end);
+  }
+  if (a->exponent < 100 || (a->fft_length > 0 && a->fft_length < 4096)) return LL_WORK_INVALID_DATA;
+  return LL_WORK_OK;
+}
+
+int ll_is_prime(uint32_t n) {
+  if (n < 2) return 0;
+  if (n % 2 == 0) return n == 2;
+  for (uint64_t d = 3; d * d <= n; d += 2)
+    if (n % d == 0) return 0;
+  return 1;
+}
+
+int ll_valid_assignment(uint32_t p, uint32_t fft_length) {
+  if (p < 7500) {
+    fprintf(stderr, "Warning: exponents < 7500 are not supported!\n");
+    return 0;
+  }
+  if (fft_length % 1024) {
+    fprintf(stderr, "Warning: FFT length %u is invalid, it must be a multiple of 1024.\n", fft_length);
+    return 0;
+  }
+  if (!ll_is_prime(p)) {
+    fprintf(stderr, "Warning: exponent is not prime!\n");
+    return -1;
+  }
+  return 1;
+}
+
+// One line of f without its newline; false at the end of the file.  Lines
+// of any length are read whole, so a long line is one rejected line (the
+// reference's fixed buffer reads its remainder as a further line).
+static bool work_read_line(FILE *f, std::string &line) {
+  line.clear();
+  char buf[256];
+  while (fgets(buf, sizeof buf, f)) {
+    line += buf;
+    if (!line.empty() && line.back() == '\n') {
+      line.pop_back();
+      return true;
+    }
+  }
+  return !line.empty();
+}
+
+static const char *work_reason(int st) {
+  switch (st) {
+    case LL_WORK_LONG_LINE: return "line is too long.";
+    case LL_WORK_NO_TEST_EQUAL: return "doesn't begin with Test= or DoubleCheck=.";
+    case LL_WORK_INVALID_FORMAT: return "invalid format.";
+    case LL_WORK_INVALID_DATA: return "invalid data.";
+    default: return "unknown error.";
+  }
+}
+
+int ll_work_next(const char *file, ll_assignment *out) {
+  if (!file || !out) return LL_ERR_ARG;
+  FILE *f = fopen(file, "r");
+  if (!f) {
+    fprintf(stderr, "Can't open workfile %s\n\n", file);
+    return 1;
+  }
+  std::string line;
+  unsigned lineno = 0;
+  while (work_read_line(f, line)) {
+    lineno++;
+    int st = ll_work_parse_line(line.c_str(), out);
+    if (st == LL_WORK_BLANK || st == LL_WORK_COMMENT) continue;
+    if (st == LL_WORK_OK) {
+      if (ll_valid_assignment(out->exponent, out->fft_length) > 0) {
+        fclose(f);
+        return 0;
+      }
+      st = LL_WORK_INVALID_DATA;
+    }
+    fprintf(stderr, "Warning: ignoring line %u: \"%s\" in \"%s\". Reason: %s\n", lineno, line.c_str(), file,
+            work_reason(st));
+  }
+  fclose(f);
+  memset(out, 0, sizeof *out);
+  fprintf(stderr, "No valid assignment found.\n\n");
+  return 2;
+}
+
+int ll_work_clear(const char *file, uint32_t p) {
+  if (!file) return LL_ERR_ARG;
+  FILE *f = fopen(file, "r");
+  if (!f) {
+    fprintf(stderr, "Can't open workfile %s\n\n", file);
+    return 3;
+  }
+  std::vector<std::string> lines;
+  std::vector<int> status;
+  std::vector<uint32_t> expo;
+  std::string line;
+  ll_assignment a;
+  while (work_read_line(f, line)) {
+    status.push_back(ll_work_parse_line(line.c_str(), &a));
+    expo.push_back(a.exponent);
+    lines.push_back(line);
+  }
+  fclose(f);
+  // The target and the first line that goes with it (parse.c:592-615).
+  size_t target = lines.size(), first = lines.size();
+  bool blank_seen = false;
+  for (size_t i = 0; i < lines.size(); i++) {
+    if (status[i] == LL_WORK_OK) {
+      if (expo[i] == p) {
+        target = i;
+        if (first > i) first = i;
+        break;
+      }
+      first = i + 1;
+      blank_seen = true;  // an assignment above: blank lines no longer matter
+    } else if (status[i] == LL_WORK_BLANK && !blank_seen) {
+      first = i + 1;
+      blank_seen = true;
+    }
+  }
+  if (target == lines.size()) {
+    fprintf(stderr, "Assignment M%u completed but not found in workfile\n\n", p);
+    return 5;
+  }
+  const std::string tmp = std::string(file) + ".tmp";
+  FILE *o = fopen(tmp.c_str(), "w");
+  if (!o) {
+    fprintf(stderr, "Can't open tmp workfile\n\n");
+    return 4;
+  }
+  bool ok = true;
+  for (size_t i = 0; i < lines.size(); i++) {
+    const bool keep = i > target || (i < target && (status[i] == LL_WORK_OK || i < first));
+    if (keep) ok = ok && fprintf(o, "%s\n", lines[i].c_str()) >= 0;
+  }
+  ok = (fclose(o) == 0) && ok;
+  if (!ok || rename(tmp.c_str(), file) != 0) {
+    remove(tmp.c_str());
+    fprintf(stderr, "Cannot move tmp workfile to regular workfile\n\n");
+    return 6;
+  }
+  return 0;
 }
 
 }  // extern "C"

@@ -322,6 +322,46 @@ void ll_fft_bounds(uint32_t p, int32_t *lb, int32_t *ub);
 /* fft_from_str (CUDALucas.cu:1070-1094): "4096K", "4M", "4194304"; -1 on error */
 int ll_fft_from_str(const char *s);
 
+/* ---- worktodo file (parse.c:155-678) ---------------------------------- */
+/* One parsed work line (struct ASSIGNMENT, parse.c:230-236). */
+#define LL_WORK_TEXT 192
+typedef struct ll_assignment {
+  uint32_t exponent;           /* the largest plain number on the line */
+  uint32_t fft_length;         /* words, from a "<k>K" or "<m>M" field; 0 = none */
+  char key[LL_WORK_TEXT];      /* the field holding a letter a-f (hex key, N/A); "" = none */
+  char comment[LL_WORK_TEXT];  /* the text after a #, // or \\ delimiter */
+} ll_assignment;
+/* Line status (enum PARSE_WARNINGS, parse.c:218-228). */
+#define LL_WORK_OK 0
+#define LL_WORK_EOF 1
+#define LL_WORK_LONG_LINE 2      /* 191 characters or more */
+#define LL_WORK_NO_TEST_EQUAL 3  /* neither "Test=" nor "DoubleCheck=" */
+#define LL_WORK_INVALID_FORMAT 4 /* more than 5 fields, a field without a number */
+#define LL_WORK_INVALID_DATA 5   /* out of range: exponent < 100, length < 4096 */
+#define LL_WORK_BLANK 6
+#define LL_WORK_COMMENT 7        /* a line starting with #, // or \\ */
+/* parse_worktodo_line (parse.c:241-484) on one line; a trailing newline is
+ * ignored. Fills out (zeroed first) and returns an LL_WORK_* status. */
+int ll_work_parse_line(const char *line, ll_assignment *out);
+/* isprime (parse.c:155-174): 1 if n is prime. */
+int ll_is_prime(uint32_t n);
+/* valid_assignment (parse.c:176-203): 1 = valid; 0 = exponent < 7500 or a
+ * length that is not a multiple of 1024; -1 = exponent not prime. Prints the
+ * reference's warning on stderr. */
+int ll_valid_assignment(uint32_t p, uint32_t fft_length);
+/* get_next_assignment (parse.c:486-554): the first line that parses and is
+ * valid; each other non-blank, non-comment line before it is reported on
+ * stderr and skipped. Returns 0, 1 = cannot open the file, 2 = none found. */
+int ll_work_next(const char *file, ll_assignment *out);
+/* clear_assignment (parse.c:556-678): rewrites the file without the first
+ * assignment for exponent p.  The comment, blank and unparsable lines just
+ * above it go with it: back to the previous assignment line, or, with none
+ * above, back to the file's first blank line (lines before that stay).
+ * Returns 0, 3 = cannot open the file, 4 = cannot create the temporary
+ * "<file>.tmp", 5 = no such assignment (file unchanged), 6 = cannot replace
+ * the file. */
+int ll_work_clear(const char *file, uint32_t p);
+
 #ifdef __cplusplus
 }
 #endif

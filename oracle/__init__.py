@@ -67,6 +67,35 @@ def lib():
     return _lib
 
 
+_PARSE_REF_PATH = os.path.join(_HERE, "_ref", "libparse_ref.so")
+_parse_ref = None
+
+
+class RefAssignment(ctypes.Structure):
+    """struct ASSIGNMENT of the reference's parse.c (parse.c:230-236)."""
+    _fields_ = [("exponent", ctypes.c_int), ("fft_length", ctypes.c_int),
+                ("hex_key", ctypes.c_char * 192), ("comment", ctypes.c_char * 192)]
+
+
+def parse_ref():
+    """The reference's own parse.c (worktodo and ini reading) as compiled in
+    place by oracle/Makefile into oracle/_ref/, or None where the reference
+    sources were not present to build it."""
+    global _parse_ref
+    if _parse_ref is None and os.path.exists(_PARSE_REF_PATH):
+        L = ctypes.CDLL(_PARSE_REF_PATH)
+        P, i32, cp = ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p
+        L.parse_worktodo_line.argtypes = [P, ctypes.POINTER(RefAssignment), P, ctypes.POINTER(P)]
+        L.get_next_assignment.argtypes = [cp, ctypes.POINTER(i32), ctypes.POINTER(i32), P]
+        L.clear_assignment.argtypes = [cp, i32]
+        L.valid_assignment.argtypes = [i32, i32]
+        L.isprime.argtypes = [ctypes.c_uint]
+        L.IniGetInt.argtypes = [cp, cp, ctypes.POINTER(i32), i32]
+        L.IniGetStr.argtypes = [cp, cp, P, cp]
+        _parse_ref = L
+    return _parse_ref
+
+
 def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
