@@ -120,15 +120,23 @@ def test_selftest_short_table(tmp_path):
 
 
 def test_worktodo_and_results(tmp_path):
+    """get_next_assignment / clear_assignment (parse.c:486-678) around check:
+    a composite exponent's line is skipped with the reference's warning and
+    stays; a length on the work line (8K, inside 86243's bounds) holds for
+    that test only."""
     aid = "0123456789ABCDEF0123456789ABCDEF"
     (tmp_path / "worktodo.txt").write_text(
-        "Test=N/A,%d,69,0\n# comment\nDoubleCheck=%s,44497,68,1\n" % (W.SMALL_P, aid))
+        "Test=N/A,%d,69,0\n# comment\nTest=N/A,44499,69,0\nDoubleCheck=%s,86243,8K,68,1\n" % (W.SMALL_P, aid))
+    assert L.choose_fft_length(86243)[0] == 4096  # the default the work line overrides
     r = drive(["-x", "10000"], tmp_path)
     assert r.returncode == 2  # stops when no assignment is left
+    assert "Warning: exponent is not prime!" in r.stderr and "No valid assignment found." in r.stderr
     lines = (tmp_path / "results.txt").read_text().strip().splitlines()
     assert lines[0].startswith("M( %d )C, 0x%016x" % (W.SMALL_P, gmp_res64(W.SMALL_P, W.SMALL_P - 2)))
-    assert lines[1].startswith("M( 44497 )P") and lines[1].endswith("AID: " + aid)
-    assert (tmp_path / "worktodo.txt").read_text() == "# comment\n"
+    assert lines[0].split("n = ")[1].startswith("%dK" % (L.choose_fft_length(W.SMALL_P)[0] // 1024))
+    assert lines[1].startswith("M( 86243 )P") and lines[1].endswith("AID: " + aid)
+    assert lines[1].split("n = ")[1].startswith("8K")
+    assert (tmp_path / "worktodo.txt").read_text() == "# comment\nTest=N/A,44499,69,0\n"
 
 
 def test_sigint_writes_checkpoint(tmp_path):
