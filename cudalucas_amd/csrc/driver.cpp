@@ -105,7 +105,13 @@ static int check_interval(int v) {
 }
 
 // ------------------------------------------------------------------- ini
-static bool ini_get(const std::string &file, const char *key, std::string &out) {
+// IniGetInt / IniGetInt2 / IniGetStr (parse.c:683-771): the value of the
+// first line "<key>=<value>" that `accept` takes (a number, two numbers, a
+// non-empty string); a line whose value it refuses is passed over for a
+// later one, and with none the caller's default holds.  The newline and one
+// carriage return before it are not part of the value.
+static bool ini_get(const std::string &file, const char *key, std::string &out,
+                    bool (*accept)(const std::string &)) {
   FILE *f = fopen(file.c_str(), "r");
   if (!f) return false;
   char buf[1024];
@@ -114,25 +120,39 @@ static bool ini_get(const std::string &file, const char *key, std::string &out) 
   while (!found && fgets(buf, sizeof buf, f)) {
     if (!strncmp(buf, key, kl) && buf[kl] == '=') {
       std::string v(buf + kl + 1);
-      while (!v.empty() && (v.back() == '\n' || v.back() == '\r')) v.pop_back();
-      out = v;
-      found = true;
+      if (!v.empty() && v.back() == '\n') v.pop_back();
+      if (!v.empty() && v.back() == '\r') v.pop_back();
+      if (accept(v)) {
+        out = v;
+        found = true;
+      }
     }
   }
   fclose(f);
   return found;
 }
 
+static bool ini_is_int(const std::string &v) {
+  int x;
+  return sscanf(v.c_str(), "%d", &x) == 1;
+}
+static bool ini_is_int_pair(const std::string &v) {
+  int x, y;
+  return sscanf(v.c_str(), "%d %d", &x, &y) == 2;
+}
+static bool ini_is_text(const std::string &v) { return !v.empty(); }
+
 static void ini_int(const std::string &file, const char *key, int &dst, int dflt) {
   if (dst >= 0) return;  // command line wins
   std::string v;
-  dst = (ini_get(file, key, v) && !v.empty()) ? atoi(v.c_str()) : dflt;
+  dst = dflt;
+  if (ini_get(file, key, v, ini_is_int)) sscanf(v.c_str(), "%d", &dst);
 }
 
 static void ini_str(const std::string &file, const char *key, std::string &dst, const char *dflt) {
   if (!dst.empty()) return;
   std::string v;
-  dst = (ini_get(file, key, v) && !v.empty()) ? v : std::string(dflt);
+  dst = ini_get(file, key, v, ini_is_text) ? v : std::string(dflt);
 }
 
 // process_options (CUDALucas.cu:3388-3522)
@@ -147,11 +167,11 @@ static void process_options(Config &c) {
   ini_int(c.ini, "ErrorReset", c.er, 85);
   if (c.threads[0] <= 0) {
     std::string v;
-    if (ini_get(c.ini, "Threads", v)) sscanf(v.c_str(), "%d %d", &c.threads[0], &c.threads[1]);
+    if (ini_get(c.ini, "Threads", v, ini_is_int_pair)) sscanf(v.c_str(), "%d %d", &c.threads[0], &c.threads[1]);
   }
   if (c.fftlen < 0) {
     std::string v;
-    c.fftlen = (ini_get(c.ini, "FFTLength", v) && !v.empty()) ? ll_fft_from_str(v.c_str()) : 0;
+    c.fftlen = ini_get(c.ini, "FFTLength", v, ini_is_text) ? ll_fft_from_str(v.c_str()) : 0;
     if (c.fftlen < 0) c.fftlen = 0;
   }
   ini_str(c.ini, "WorkFile", c.workfile, "worktodo.txt");

@@ -271,3 +271,40 @@ def test_driver_bench_options_parse(tmp_path):
     assert "-cufftbench" in r.stderr and "-threadbench" in r.stderr and "--selector" in r.stderr
     r = run_driver(["-cufftbench", "1"], tmp_path)
     assert r.returncode == 2 and "can't parse -cufftbench" in r.stderr
+
+
+@pytest.mark.skipif(O.parse_ref() is None, reason="reference parse.c not built (oracle/_ref absent)")
+def test_driver_ini_reading_matches_reference_readers(tmp_path):
+    """process_options' ini reads (IniGetInt / IniGetStr, parse.c:683-771)
+    against the reference's own readers on seeded random ini files:
+    duplicate keys, values that do not parse (passed over for a later line),
+    empty values, CRLF endings, near-miss keys."""
+    import random
+    R = O.parse_ref()
+    rng = random.Random(3388)
+    ints = {"DeviceNumber": ("device", 0), "ErrorReset": ("error_reset", 85),
+            "SaveAllCheckpoints": ("save_all", 0), "RoundoffTest": ("roundoff", 0)}
+    strs = {"WorkFile": ("workfile", "worktodo.txt"), "ResultsFile": ("results", "results.txt"),
+            "SaveFolder": ("save_folder", "savefiles")}
+    for case in range(60):
+        lines = []
+        for _ in range(rng.randrange(0, 14)):
+            key = rng.choice(list(ints) + list(strs) + ["DeviceNumbers", "workfile", " WorkFile", "Polite"])
+            if key in strs or key.lower() == "workfile":
+                val = rng.choice(["", "w%d.txt" % rng.randrange(100), "dir/r.txt", "x"])
+            else:
+                val = rng.choice(["", str(rng.randrange(0, 200)), "abc", " 7", "12abc", "0x10", "-0"])
+            lines.append("%s%s%s%s" % (key, rng.choice(["=", "=", " ="]), val, rng.choice(["\n", "\r\n"])))
+        ini = tmp_path / ("c%d.ini" % case)
+        ini.write_text("".join(lines), newline="")
+        r = run_driver(["-i", ini.name, "--print-config"], tmp_path)
+        assert r.returncode == 0, r.stderr
+        cfg = dict(kv.split("=", 1) for kv in r.stdout.split())
+        for key, (name, dflt) in ints.items():
+            v = ctypes.c_int(0)
+            R.IniGetInt(os.fsencode(ini), key.encode(), ctypes.byref(v), dflt)
+            assert int(cfg[name]) == v.value, (case, key, lines)
+        for key, (name, dflt) in strs.items():
+            buf = ctypes.create_string_buffer(192)
+            R.IniGetStr(os.fsencode(ini), key.encode(), buf, dflt.encode())
+            assert cfg[name] == buf.value.decode(), (case, key, lines)
