@@ -167,6 +167,9 @@ def lib():
     L.ll_valid_assignment.argtypes = [u32, u32]
     L.ll_work_next.argtypes = [ctypes.c_char_p, ctypes.POINTER(Assignment)]
     L.ll_work_clear.argtypes = [ctypes.c_char_p, u32]
+    L.ll_fopen_locked.argtypes = [ctypes.c_char_p, ctypes.c_char_p, u32]
+    L.ll_fopen_locked.restype = P
+    L.ll_fclose_locked.argtypes = [P]
     _lib = L
     return L
 

@@ -490,7 +490,7 @@ static int check(uint32_t q, const Config &cfg, const char *aid) {
   bool zero = true;
   for (size_t i = 0; i < (q + 31) / 32; i++) zero = zero && w[i] == 0;
   const uint64_t res = (uint64_t)w[0] | ((uint64_t)w[1] << 32);
-  FILE *fp = fopen(cfg.resultsfile.c_str(), "a");
+  FILE *fp = ll_fopen_locked(cfg.resultsfile.c_str(), "a", 0);  // fopen_and_lock (CUDALucas.cu:3149)
   if (zero) {
     printf("M( %u )P, n = %dK, %s", q, n / 1024, kProgram);
     if (fp) fprintf(fp, "M( %u )P, offset = %u, n = %dK, %s", q, offset, n / 1024, kProgram);
@@ -505,7 +505,7 @@ static int check(uint32_t q, const Config &cfg, const char *aid) {
   if (fp) {
     if (aid && *aid && strncasecmp(aid, "N/A", 3)) fprintf(fp, ", AID: %s\n", aid);
     else fprintf(fp, "\n");
-    fclose(fp);
+    ll_fclose_locked(fp);
   } else {
     fprintf(stderr, "Cannot write results to %s\n\n", cfg.resultsfile.c_str());
     return 1;
@@ -666,7 +666,7 @@ static void print_device_info(int device) {
 // init_ffts (CUDALucas.cu:947-1025): the lengths listed in fft.txt, with
 // the defaults below its first and above its last entry.
 static void load_fft_file(const std::string &path) {
-  FILE *f = path.empty() ? nullptr : fopen(path.c_str(), "r");
+  FILE *f = path.empty() ? nullptr : ll_fopen_locked(path.c_str(), "r", 0);
   if (!f) return;
   std::vector<uint32_t> file;
   char buf[256];
@@ -674,7 +674,7 @@ static void load_fft_file(const std::string &path) {
     const int k = atoi(buf);
     if (k > 0 && (file.empty() || (uint32_t)k * 1024u > file.back())) file.push_back((uint32_t)k * 1024u);
   }
-  fclose(f);
+  ll_fclose_locked(f);
   if (file.empty()) return;
   ll_fft_length_list_set(0, nullptr);
   std::vector<uint32_t> merged;
@@ -690,7 +690,7 @@ static void load_fft_file(const std::string &path) {
 // init_threads reading threads.txt (CUDALucas.cu:812-945), for schedules:
 // lines "K path W H R ms", a later line for the same length wins.
 static void load_schedule_file(const std::string &path) {
-  FILE *f = path.empty() ? nullptr : fopen(path.c_str(), "r");
+  FILE *f = path.empty() ? nullptr : ll_fopen_locked(path.c_str(), "r", 0);
   if (!f) return;
   std::vector<ll_schedule> t;
   char buf[256];
@@ -707,7 +707,7 @@ static void load_schedule_file(const std::string &path) {
       }
     if (!replaced) t.push_back(e);
   }
-  fclose(f);
+  ll_fclose_locked(f);
   ll_schedule_table_set((uint32_t)t.size(), t.data());
 }
 
@@ -797,7 +797,7 @@ static int cufftbench(const Config &cfg, int s, int e, int passes) {
   const std::string path = device_file(cfg.device, "fft.txt");
   ll_device_info d;
   ll_get_device_info(cfg.device < 0 ? 0 : cfg.device, &d);
-  FILE *f = path.empty() ? nullptr : fopen(path.c_str(), "w");
+  FILE *f = path.empty() ? nullptr : ll_fopen_locked(path.c_str(), "w", 0);
   FILE *out = f ? f : stdout;
   if (!f) printf("Cannot open %s.\n", path.c_str());
   fprintf(out, "Device              %s\n", d.name);
@@ -809,7 +809,7 @@ static int cufftbench(const Config &cfg, int s, int e, int passes) {
     if (keep[i])
       fprintf(out, "%5d %10u %8.4f\n", t[i].first, max_exponent((uint32_t)t[i].first * 1024u), t[i].second);
   if (f) {
-    fclose(f);
+    ll_fclose_locked(f);
     printf("Optimal fft lengths saved in %s.\n", path.c_str());
   }
   return 0;
@@ -826,7 +826,7 @@ static int threadbench(const Config &cfg, int s, int e, int passes, int mode) {
   }
   printf("Schedule bench, testing every schedule for ffts %dK to %dK, doing %d passes.\n", s, e, passes);
   const std::string path = device_file(cfg.device, "schedules.txt");
-  FILE *f = path.empty() ? nullptr : fopen(path.c_str(), "a");
+  bool appended = false;
   for (int k = s; k <= e; k++) {
     const uint32_t n = (uint32_t)k * 1024u;
     if (mode & 1) {
@@ -850,12 +850,15 @@ static int threadbench(const Config &cfg, int s, int e, int passes, int mode) {
            v[0].width, v[0].height, v[0].lines);
     if (!(mode & 8)) printf("\n");
     fflush(stdout);
-    if (f) fprintf(f, "%5d %u %u %u %u %10.5f\n", k, v[0].path, v[0].width, v[0].height, v[0].lines, v[0].ms);
+    // the file is locked only while a line is appended (CUDALucas.cu:2269-2274)
+    FILE *f = path.empty() ? nullptr : ll_fopen_locked(path.c_str(), "a", 0);
+    if (f) {
+      fprintf(f, "%5d %u %u %u %u %10.5f\n", k, v[0].path, v[0].width, v[0].height, v[0].lines, v[0].ms);
+      ll_fclose_locked(f);
+      appended = true;
+    }
   }
-  if (f) {
-    fclose(f);
-    printf("Best schedules appended to %s.\n", path.c_str());
-  }
+  if (appended) printf("Best schedules appended to %s.\n", path.c_str());
   return 0;
 }
 

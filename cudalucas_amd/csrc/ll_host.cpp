@@ -3,6 +3,7 @@
 // extended-precision tables (roots of unity, IBDWT weights) the kernels read.
 #include <ctype.h>
 #include <errno.h>
+#include <fcntl.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -10,7 +11,10 @@
 #include <strings.h>
 #include <sys/stat.h>
 #include <sys/types.h>
+#include <time.h>
+#include <unistd.h>
 
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -658,6 +662,81 @@ int ll_work_clear(const char *file, uint32_t p) {
     return 6;
   }
   return 0;
+}
+
+// ----------------------------------------------------------- locked files
+
+namespace {
+struct HeldLock {
+  FILE *file;
+  int fd;  // -1: the file was opened without a lock
+  std::string lock_path;
+};
+std::mutex g_lock_mutex;
+std::vector<HeldLock> g_held;
+}  // namespace
+
+FILE *ll_fopen_locked(const char *path, const char *mode, uint32_t wait_ms) {
+  if (!path || !mode) {
+    errno = EINVAL;
+    return nullptr;
+  }
+  const std::string lock = std::string(path) + ".lck";
+  int fd;
+  unsigned sleep_ms = 0;
+  uint64_t waited_ms = 0;
+  for (;;) {
+    fd = open(lock.c_str(), O_CREAT | O_EXCL | O_WRONLY, 0644);
+    if (fd >= 0) break;
+    if (errno != EEXIST) {
+      perror("Cannot open lockfile");
+      break;
+    }
+    if (wait_ms && waited_ms >= wait_ms) {
+      errno = ETIMEDOUT;
+      return nullptr;
+    }
+    if (sleep_ms == 0) fprintf(stderr, "%s is locked, waiting ...\n", path);
+    if (sleep_ms < 1000) sleep_ms++;
+    const struct timespec ts = {(time_t)(sleep_ms / 1000), (long)(sleep_ms % 1000) * 1000000L};
+    nanosleep(&ts, nullptr);
+    waited_ms += sleep_ms;
+  }
+  if (fd >= 0 && sleep_ms > 0) printf("Locked %s\n", path);
+  FILE *f = fopen(path, mode);
+  if (!f) {
+    const int err = errno;
+    if (fd >= 0) {
+      close(fd);
+      remove(lock.c_str());
+    }
+    errno = err;
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> g(g_lock_mutex);
+  g_held.push_back({f, fd, lock});
+  return f;
+}
+
+int ll_fclose_locked(FILE *f) {
+  if (!f) return EOF;
+  HeldLock held{nullptr, -1, ""};
+  {
+    std::lock_guard<std::mutex> g(g_lock_mutex);
+    for (size_t i = 0; i < g_held.size(); i++)
+      if (g_held[i].file == f) {
+        held = g_held[i];
+        g_held.erase(g_held.begin() + (long)i);
+        break;
+      }
+  }
+  if (!held.file) fprintf(stderr, "File was not locked!\n");
+  const int rc = fclose(f);
+  if (held.fd >= 0) {
+    if (close(held.fd) != 0) perror("Failed to close lockfile");
+    if (remove(held.lock_path.c_str()) != 0) perror("Failed to delete lockfile");
+  }
+  return rc;
 }
 
 }  // extern "C"

@@ -24,6 +24,7 @@
 #define LL_API_H
 #include <stddef.h>
 #include <stdint.h>
+#include <stdio.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -361,6 +362,18 @@ int ll_work_next(const char *file, ll_assignment *out);
  * "<file>.tmp", 5 = no such assignment (file unchanged), 6 = cannot replace
  * the file. */
 int ll_work_clear(const char *file, uint32_t p);
+
+/* ---- locked files (fopen_and_lock / unlock_and_fclose, parse.c:775-894) - */
+/* Opens path with a lock: "<path>.lck", created exclusively, is the lock,
+ * so this build and the reference exclude each other.  A lock held
+ * elsewhere is waited for ("<path> is locked, waiting ..." once, sleeps
+ * growing to 1 s); wait_ms = 0 waits as long as it takes, as the reference
+ * does, otherwise NULL with errno = ETIMEDOUT after about wait_ms.  Where the
+ * lock file cannot be made at all (not an existing lock) the file is opened
+ * without one, as the reference does.  NULL if fopen fails. */
+FILE *ll_fopen_locked(const char *path, const char *mode, uint32_t wait_ms);
+/* Closes a file from ll_fopen_locked and removes its lock; fclose's result. */
+int ll_fclose_locked(FILE *f);
 
 #ifdef __cplusplus
 }

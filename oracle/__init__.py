@@ -92,6 +92,9 @@ def parse_ref():
         L.isprime.argtypes = [ctypes.c_uint]
         L.IniGetInt.argtypes = [cp, cp, ctypes.POINTER(i32), i32]
         L.IniGetStr.argtypes = [cp, cp, P, cp]
+        L.fopen_and_lock.argtypes = [cp, cp]
+        L.fopen_and_lock.restype = P
+        L.unlock_and_fclose.argtypes = [P]
         _parse_ref = L
     return _parse_ref
 

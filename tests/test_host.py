@@ -308,3 +308,41 @@ def test_driver_ini_reading_matches_reference_readers(tmp_path):
             buf = ctypes.create_string_buffer(192)
             R.IniGetStr(os.fsencode(ini), key.encode(), buf, dflt.encode())
             assert cfg[name] == buf.value.decode(), (case, key, lines)
+
+
+def test_locked_files_wait_time_out_and_exclude_the_reference(tmp_path, capfd):
+    """fopen_and_lock / unlock_and_fclose (parse.c:775-894): "<path>.lck" is
+    the lock; a holder is waited for, and the reference's own lock (where it
+    is built) excludes this build's and the other way round."""
+    import threading
+    import time
+    lib = L.lib()
+    path = os.fsencode(tmp_path / "results.txt")
+    lck = tmp_path / "results.txt.lck"
+    f = lib.ll_fopen_locked(path, b"a", 0)
+    assert f and lck.exists()
+    t0 = time.monotonic()
+    assert not lib.ll_fopen_locked(path, b"a", 60)  # held: gives up after about 60 ms
+    assert time.monotonic() - t0 >= 0.05
+    # a waiter without a limit gets the file once the holder lets go
+    threading.Timer(0.15, lambda: lib.ll_fclose_locked(f)).start()
+    g = lib.ll_fopen_locked(path, b"a", 0)
+    assert g and lck.exists()
+    assert lib.ll_fclose_locked(g) == 0 and not lck.exists()
+    libc = ctypes.CDLL(None)
+    libc.fflush.argtypes = [ctypes.c_void_p]
+    libc.fflush(None)  # the messages go through C stdio
+    out = capfd.readouterr()
+    assert "results.txt is locked, waiting ..." in out.err and "Locked " in out.out
+    R = O.parse_ref()
+    if R is None:
+        return
+    h = R.fopen_and_lock(path, b"a")
+    assert h and lck.exists()
+    assert not lib.ll_fopen_locked(path, b"a", 60)
+    R.unlock_and_fclose(h)
+    assert not lck.exists()
+    f = lib.ll_fopen_locked(path, b"a", 60)
+    assert f
+    lib.ll_fclose_locked(f)
+    assert not lck.exists()
