@@ -922,6 +922,10 @@ int main(int argc, char **argv) {
     }
     if (!strcmp(a, "-info")) { cfg.info = true; continue; }
     if (!strcmp(a, "-k")) continue;
+    if (!strcmp(a, "-memtest")) {  // CUDALucas.cu:2295-2411, a VRAM burn-in
+      fprintf(stderr, "-memtest is not supported: it tests the card's memory, not LL arithmetic\n\n");
+      return 2;
+    }
     if (!strcmp(a, "-cufftbench")) {
       if (!need(argc, argv, i, 3)) { fprintf(stderr, "can't parse -cufftbench option\n\n"); return 2; }
       for (int k = 0; k < 3; k++) cfg.cufftbench[k] = atoi(argv[++i]);

@@ -271,6 +271,8 @@ def test_driver_bench_options_parse(tmp_path):
     assert "-cufftbench" in r.stderr and "-threadbench" in r.stderr and "--selector" in r.stderr
     r = run_driver(["-cufftbench", "1"], tmp_path)
     assert r.returncode == 2 and "can't parse -cufftbench" in r.stderr
+    r = run_driver(["-memtest", "10", "1"], tmp_path)
+    assert r.returncode == 2 and "-memtest is not supported" in r.stderr
 
 
 @pytest.mark.skipif(O.parse_ref() is None, reason="reference parse.c not built (oracle/_ref absent)")
