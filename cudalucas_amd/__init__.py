@@ -167,6 +167,7 @@ def lib():
     L.ll_valid_assignment.argtypes = [u32, u32]
     L.ll_work_next.argtypes = [ctypes.c_char_p, ctypes.POINTER(Assignment)]
     L.ll_work_clear.argtypes = [ctypes.c_char_p, u32]
+    L.ll_format_report.argtypes = [ctypes.c_char_p, ctypes.POINTER(Report), ctypes.c_char_p, ctypes.c_size_t]
     L.ll_fopen_locked.argtypes = [ctypes.c_char_p, ctypes.c_char_p, u32]
     L.ll_fopen_locked.restype = P
     L.ll_fclose_locked.argtypes = [P]
@@ -283,6 +284,25 @@ class Assignment(ctypes.Structure):
     """ll_assignment: one parsed work line (struct ASSIGNMENT, parse.c:230-236)."""
     _fields_ = [("exponent", ctypes.c_uint32), ("fft_length", ctypes.c_uint32),
                 ("key", ctypes.c_char * 192), ("comment", ctypes.c_char * 192)]
+
+
+class Report(ctypes.Structure):
+    """ll_report: the fields of one report line (process_output,
+    CUDALucas.cu:1633-1701)."""
+    _fields_ = [("p", ctypes.c_uint32), ("n", ctypes.c_uint32), ("iteration", ctypes.c_uint32),
+                ("offset", ctypes.c_uint32), ("residue", ctypes.c_uint64), ("max_err", ctypes.c_float),
+                ("interval_iterations", ctypes.c_uint32), ("interval_us", ctypes.c_uint64),
+                ("eta_s", ctypes.c_uint64), ("now", ctypes.c_int64), ("program", ctypes.c_char_p)]
+
+
+def format_report(fmt, **fields):
+    """ll_format_report: one report line by an OutputString format."""
+    prog = fields.pop("program", "CUDALucasAMD")
+    r = Report(program=prog.encode(), **fields)
+    need = lib().ll_format_report(fmt.encode(), ctypes.byref(r), None, 0)
+    buf = ctypes.create_string_buffer(need + 1)
+    lib().ll_format_report(fmt.encode(), ctypes.byref(r), buf, need + 1)
+    return buf.value.decode()
 
 
 def work_parse_line(line):

@@ -36,6 +36,8 @@ struct Config {
   int selftest = -1;
   int roundoff = -1;
   std::string workfile, resultsfile;
+  std::string output_string, output_header;  // OutputString, OutputHeader
+  int output_interval = -1;                  // OutputHInterval; 0 = the one-line report
   // options without a reference counterpart (tests and tooling)
   std::string ckpt_dir;
   int64_t fixed_offset = -1;
@@ -177,6 +179,10 @@ static void process_options(Config &c) {
   ini_str(c.ini, "WorkFile", c.workfile, "worktodo.txt");
   ini_str(c.ini, "ResultsFile", c.resultsfile, "results.txt");
   ini_str(c.ini, "SaveFolder", c.save_folder, "savefiles");
+  ini_str(c.ini, "OutputString", c.output_string, "0");
+  ini_str(c.ini, "OutputHeader", c.output_header, "0");
+  ini_int(c.ini, "OutputHInterval", c.output_interval, 0);
+  if (c.output_interval < 0) c.output_interval = 15;  // CUDALucas.cu:3488
   const int k1 = check_interval(c.cpi), k2 = check_interval(c.ri), k3 = check_interval(c.ei);
   if (k1 != c.cpi) printf("CheckpointIterations = %d must have the form k*10^m for k = 1, 2, or 5.\nChanging to %d.\n", c.cpi, k1);
   if (k2 != c.ri) printf("ReportIterations = %d must have the form k*10^m for k = 1, 2, or 5.\nChanging to %d.\n", c.ri, k2);
@@ -267,6 +273,41 @@ static int round_off_test(Engine &e, uint32_t iters) {
 
 // One LL test of 2^q - 1 (check, CUDALucas.cu:2862-3186).  Returns 0 on a
 // finished or cleanly interrupted test, 2 on an unrecoverable error.
+// process_output (CUDALucas.cu:1633-1701): the report as a table row by
+// OutputString, with OutputHeader above the first row and then every
+// OutputHInterval - 1 rows (the reference's count, kept across tests).
+static int g_header_countdown = 0;
+
+static void report_table(const Config &cfg, uint32_t q, uint32_t n, uint32_t iteration, uint32_t offset,
+                         uint64_t res, float maxerr, uint32_t iters, double seconds, uint64_t eta_s) {
+  ll_report r;
+  memset(&r, 0, sizeof r);
+  r.p = q;
+  r.n = n;
+  r.iteration = iteration;
+  r.offset = offset;
+  r.residue = res;
+  r.max_err = maxerr;
+  r.interval_iterations = iters;
+  r.interval_us = (uint64_t)(seconds * 1e6);
+  r.eta_s = eta_s;
+  r.now = (int64_t)time(nullptr);
+  r.program = kProgram;
+  std::string line(256, '\0');
+  const int len = ll_format_report(cfg.output_string.c_str(), &r, &line[0], line.size());
+  if (len >= (int)line.size()) {
+    line.assign((size_t)len + 1, '\0');
+    ll_format_report(cfg.output_string.c_str(), &r, &line[0], line.size());
+  }
+  line.resize(len > 0 ? (size_t)len : 0);
+  if (g_header_countdown == 0 && cfg.output_interval > 0) {
+    g_header_countdown = cfg.output_interval - 1;
+    printf("%s\n", cfg.output_header.c_str());
+  }
+  g_header_countdown--;
+  printf("%s\n", line.c_str());
+}
+
 static int check(uint32_t q, const Config &cfg, const char *aid) {
   const uint32_t last = q - 2;
   const double error_limit = (cfg.el - cfg.el / 8.0 * log((double)cfg.ei) / log(100.0)) / 100.0;
@@ -443,12 +484,17 @@ static int check(uint32_t q, const Config &cfg, const char *aid) {
     }
     if (stop < last) {
       const double ms = dt * 1000.0 / (stop - last_chk);
-      printf("Iteration %u / %u, 0x%016llx, %dK, %s, error = %5.5f, real: ", stop, q,
-             (unsigned long long)res, n / 1024, kProgram, maxerr);
-      print_time((unsigned)dt, 0);
-      printf(", %4.4f ms/iter, ETA: ", ms);
-      print_time((unsigned)(ms * (last - stop) / 1000.0), 2);
-      printf(", %5.2f%%\n", (float)stop / q * 100);
+      if (cfg.output_interval) {
+        report_table(cfg, q, (uint32_t)n, stop, offset, res, maxerr, stop - last_chk, dt,
+                     (uint64_t)(ms * (last - stop) / 1000.0));
+      } else {
+        printf("Iteration %u / %u, 0x%016llx, %dK, %s, error = %5.5f, real: ", stop, q,
+               (unsigned long long)res, n / 1024, kProgram, maxerr);
+        print_time((unsigned)dt, 0);
+        printf(", %4.4f ms/iter, ETA: ", ms);
+        print_time((unsigned)(ms * (last - stop) / 1000.0), 2);
+        printf(", %5.2f%%\n", (float)stop / q * 100);
+      }
       fflush(stdout);
       // illegal residues (CUDALucas.cu:3006-3016)
       if (res == 0) {
@@ -969,9 +1015,10 @@ int main(int argc, char **argv) {
   process_options(cfg);
   if (print_config) {
     printf("device=%d threads=%d,%d checkpoint=%d report=%d error_iter=%d error_limit=%d error_reset=%d "
-           "fft=%d save_all=%d save_folder=%s roundoff=%d workfile=%s results=%s\n",
+           "fft=%d save_all=%d save_folder=%s roundoff=%d workfile=%s results=%s output_interval=%d\n",
            cfg.device, cfg.threads[0], cfg.threads[1], cfg.cpi, cfg.ri, cfg.ei, cfg.el, cfg.er, cfg.fftlen,
-           cfg.save_all, cfg.save_folder.c_str(), cfg.roundoff, cfg.workfile.c_str(), cfg.resultsfile.c_str());
+           cfg.save_all, cfg.save_folder.c_str(), cfg.roundoff, cfg.workfile.c_str(), cfg.resultsfile.c_str(),
+           cfg.output_interval);
     return 0;
   }
   // handle SIGINT/SIGTERM even if the parent started us with them blocked

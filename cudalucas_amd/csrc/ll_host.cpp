@@ -664,6 +664,98 @@ int ll_work_clear(const char *file, uint32_t p) {
   return 0;
 }
 
+// ------------------------------------------------------------ report line
+
+// print_time_from_seconds (CUDALucas.cu:1429-1463): m:ss, h:mm:ss, or with
+// days d:hh:mm:ss, right-aligned in 12 characters.
+static std::string report_clock(uint64_t sec, bool days) {
+  char t[48], out[64];
+  const uint64_t d = days ? sec / 86400 : 0;
+  if (days) sec %= 86400;
+  const unsigned h = (unsigned)(sec / 3600), m = (unsigned)(sec % 3600 / 60), s = (unsigned)(sec % 60);
+  if (d) snprintf(t, sizeof t, "%llu:%02u:%02u:%02u", (unsigned long long)d, h, m, s);
+  else if (h) snprintf(t, sizeof t, "%u:%02u:%02u", h, m, s);
+  else snprintf(t, sizeof t, "%u:%02u", m, s);
+  snprintf(out, sizeof out, "%12s", t);
+  return out;
+}
+
+int ll_format_report(const char *format, const ll_report *r, char *out, size_t cap) {
+  if (!format || !r) return LL_ERR_ARG;
+  const time_t now = (time_t)r->now;
+  struct tm tm_now;
+  localtime_r(&now, &tm_now);
+  const double iters = r->interval_iterations ? (double)r->interval_iterations : 1.0;
+  std::string line;
+  char f[96];
+  for (const char *c = format; *c;) {
+    if (*c != '%') {
+      line += *c++;
+      continue;
+    }
+    const char code = c[1];
+    if (!code) break;
+    c += 2;
+    f[0] = 0;
+    double v = 0;
+    const char *fixed = nullptr;  // a %.5f field: its printf format
+    switch (code) {
+      case 'a': strftime(f, sizeof f, "%a", &tm_now); break;
+      case 'M': strftime(f, sizeof f, "%b", &tm_now); break;
+      case 'D': strftime(f, sizeof f, "%d", &tm_now); break;
+      case 'h': strftime(f, sizeof f, "%H", &tm_now); break;
+      case 'm': strftime(f, sizeof f, ":%M", &tm_now); break;
+      case 's': strftime(f, sizeof f, ":%S", &tm_now); break;
+      case 'y': strftime(f, sizeof f, "%y", &tm_now); break;
+      case 'z': strftime(f, sizeof f, "%Z", &tm_now); break;
+      case 'p': snprintf(f, sizeof f, "%9u", r->p); break;
+      case 'P': {
+        char m[24];
+        snprintf(m, sizeof m, "M%u", r->p);
+        snprintf(f, sizeof f, "%10s", m);
+        break;
+      }
+      case 'f': snprintf(f, sizeof f, "%5uK", r->n / 1024); break;
+      case 'F': snprintf(f, sizeof f, "%8u", r->n); break;
+      case 'i': snprintf(f, sizeof f, "%9u", r->iteration); break;
+      case 'o': snprintf(f, sizeof f, "%9u", r->offset); break;
+      case 'r': snprintf(f, sizeof f, "0x%016llx", (unsigned long long)r->residue); break;
+      case 'R': snprintf(f, sizeof f, "0x%016llX", (unsigned long long)r->residue); break;
+      case 'C': snprintf(f, sizeof f, "%s", report_clock(r->interval_us / 1000000, false).c_str()); break;
+      case 'T': snprintf(f, sizeof f, "%s", report_clock(r->eta_s, true).c_str()); break;
+      case 'N': snprintf(f, sizeof f, "%s", r->program ? r->program : ""); break;
+      case '%': snprintf(f, sizeof f, "%%"); break;
+      case 'x': fixed = "%7.5f", v = r->max_err; break;
+      case 'q': fixed = "%9.5f", v = r->interval_us / 1000.0 / iters; break;
+      case 'c': fixed = "%10.5f", v = r->interval_us / 1000000.0; break;
+      case 'Q': fixed = "%10.5f", v = iters * 1000000.0 / (double)r->interval_us; break;
+      case 'd': fixed = "%10.5f", v = r->iteration / (float)(r->p - 2) * 100; break;
+      default: break;  // an unknown code prints nothing
+    }
+    if (fixed) {
+      // The digit after the code keeps that many decimals (0 drops the
+      // point too); anything else keeps all five.  The character is taken
+      // either way (CUDALucas.cu:3286-3333).
+      int keep = 5;
+      if (*c) {
+        const int k = *c++ - '0';
+        if (k >= 0 && k <= 5) keep = k;
+      }
+      snprintf(f, sizeof f, fixed, v);
+      const size_t cut = (size_t)(5 - keep + (keep == 0));
+      const size_t len = strlen(f);
+      f[len > cut ? len - cut : 0] = 0;
+    }
+    line += f;
+  }
+  if (out && cap) {
+    const size_t k = line.size() < cap - 1 ? line.size() : cap - 1;
+    memcpy(out, line.data(), k);
+    out[k] = 0;
+  }
+  return (int)line.size();
+}
+
 // ----------------------------------------------------------- locked files
 
 namespace {

@@ -363,6 +363,29 @@ int ll_work_next(const char *file, ll_assignment *out);
  * the file. */
 int ll_work_clear(const char *file, uint32_t p);
 
+/* ---- the report line (process_output + encode_output_options,
+ * CUDALucas.cu:1633-1701, 3189-3372) ------------------------------------ */
+typedef struct ll_report {
+  uint32_t p, n, iteration, offset;
+  uint64_t residue;              /* Res64 */
+  float max_err;                 /* maximum roundoff since the last report */
+  uint32_t interval_iterations;  /* iterations since the last report */
+  uint64_t interval_us;          /* their wall time, microseconds */
+  uint64_t eta_s;                /* estimated seconds to the end of the test */
+  int64_t now;                   /* time_t of the report, for the date and clock */
+  const char *program;           /* program name and version */
+} ll_report;
+/* Formats one report line by the ini's OutputString: %a %M %D %h %m %s %y %z
+ * (weekday, month, day, hour, :minutes, :seconds, year, zone; local time),
+ * %p %P %f %F %i %o %r %R %N %% (exponent, M<p>, length in K, length,
+ * iteration, offset, Res64 lower/upper case, program, %), %C %T (interval
+ * h:mm:ss, ETA d:hh:mm:ss, right-aligned in 12), and %x %q %c %Q %d followed by
+ * a digit k = 0..5 (roundoff, ms/iteration, interval seconds, iterations/s,
+ * percent done, printed with 5 decimals and cut to k).  Other text is
+ * copied.  Writes at most cap bytes (NUL-terminated) and returns the length
+ * of the whole line, as snprintf does. */
+int ll_format_report(const char *format, const ll_report *r, char *out, size_t cap);
+
 /* ---- locked files (fopen_and_lock / unlock_and_fclose, parse.c:775-894) - */
 /* Opens path with a lock: "<path>.lck", created exclusively, is the lock,
  * so this build and the reference exclude each other.  A lock held

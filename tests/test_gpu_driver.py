@@ -62,6 +62,24 @@ def test_composite_run_matches_gmp_and_reports(tmp_path):
     assert not (tmp_path / ("c%d" % p)).exists()
 
 
+def test_report_table_rows_and_header(tmp_path):
+    """process_output (CUDALucas.cu:1633-1701) in a run: OutputString rows
+    with the Res64 of their iteration, the header above rows 1 and 3 of 4
+    with OutputHInterval=3."""
+    p = W.SMALL_P
+    (tmp_path / "CUDALucas.ini").write_text(
+        "OutputHInterval=3\nOutputHeader=HEAD\nOutputString=| %P %i %r | %f %x5 | %d2%% |\n")
+    r = drive(["-x", "10000", "--offset", "77", p], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = [l for l in r.stdout.splitlines() if l == "HEAD" or l.startswith("| ")]
+    rows = [l.split() for l in lines if l != "HEAD"]
+    assert [int(x[2]) for x in rows] == list(range(10000, p - 2, 10000))
+    assert [i for i, l in enumerate(lines) if l == "HEAD"] == [0, 3]
+    for x in rows:
+        assert x[1] == "M%d" % p and int(x[3], 16) == gmp_res64(p, int(x[2]))
+        assert x[5] == "%dK" % (L.choose_fft_length(p)[0] // 1024) and float(x[6]) < 0.4 and x[8].endswith("%")
+
+
 def test_stop_and_resume_from_own_checkpoint(tmp_path):
     p = W.SMALL_P
     r1 = drive(["-f", "4096", "-x", "5000", "-c", "5000", "--offset", "7", "--stop-after", "12000", p], tmp_path)

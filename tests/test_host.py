@@ -348,3 +348,59 @@ def test_locked_files_wait_time_out_and_exclude_the_reference(tmp_path, capfd):
     assert f
     lib.ll_fclose_locked(f)
     assert not lck.exists()
+
+
+INI_FORMAT = "|  %M %D  %h%m%s  | %P %i  %r  | %f  %x5 %q4 %c2s  | %T%d2%%  |"
+
+
+@pytest.mark.parametrize("j,residue,err,secs,eta,row", [
+    (18900000, 0x219381f578a2abf9, 0.28516, 539.92, (2, 11, 5, 11),
+     "|  Dec 15  12:46:01  |  M57885161  18900000  0x219381f578a2abf9  |  3136K  0.28516   5.3992  539.92s  |  2:11:05:11  32.65%  |"),
+    (19000000, 0x41e63fccc43ad9a5, 0.29297, 539.93, (2, 10, 50, 19),
+     "|  Dec 15  12:55:01  |  M57885161  19000000  0x41e63fccc43ad9a5  |  3136K  0.29297   5.3993  539.93s  |  2:10:50:19  32.82%  |"),
+    (19100000, 0x6e8b11826f5f2620, 0.29688, 539.98, (2, 10, 37, 4),
+     "|  Dec 15  13:04:01  |  M57885161  19100000  0x6e8b11826f5f2620  |  3136K  0.29688   5.3998  539.98s  |  2:10:37:04  32.99%  |"),
+])
+def test_report_table_reproduces_the_documented_rows(j, residue, err, secs, eta, row):
+    """The OutputString example and its three rows in CUDALucas.ini:113-144.
+    The rows were written with an ETA column one character narrower than
+    process_output's %12s (CUDALucas.cu:1460, 1678), so that one column is
+    compared with its leading blank taken off; every other character must
+    match."""
+    import time
+    clock = row.split("|")[1].split()
+    hh, mm, ss = (int(x) for x in clock[2].split(":"))
+    now = int(time.mktime((2012, 12, int(clock[1]), hh, mm, ss, 0, 0, -1)))
+    d, h, m, s = eta
+    line = L.format_report(INI_FORMAT, p=57885161, n=3136 * 1024, iteration=j, residue=residue, max_err=err,
+                           interval_iterations=100000, interval_us=int(round(secs * 1e6)),
+                           eta_s=((d * 24 + h) * 60 + m) * 60 + s, now=now)
+    assert line.replace("|   %d:" % d, "|  %d:" % d) == row
+
+
+def test_report_codes_widths_and_cuts():
+    """Each code of encode_output_options / process_output
+    (CUDALucas.cu:3189-3372, 1633-1701) with the widths the reference prints."""
+    f = dict(p=86243, n=8192, iteration=20000, offset=12345, residue=0xABCDEF0123456789, max_err=0.1875,
+             interval_iterations=10000, interval_us=2500000, eta_s=3725, now=0, program="CUDALucasAMD v1.0")
+    assert L.format_report("%p|%P|%f|%F|%i|%o", **f) == "    86243|    M86243|    8K|    8192|    20000|    12345"
+    assert L.format_report("%r %R %N %%", **f) == "0xabcdef0123456789 0xABCDEF0123456789 CUDALucasAMD v1.0 %"
+    assert L.format_report("%x5|%x3|%x0|%x9|%x", **f) == "0.18750|0.187|0|0.18750|0.18750"
+    assert L.format_report("%q4|%c2|%Q1|%d2", **f) == "  0.2500|   2.50|4000.0|  23.19"
+    assert L.format_report("[%C][%T]", **f) == "[        0:02][     1:02:05]"
+    assert L.format_report("[%T]", **dict(f, eta_s=2 * 86400 + 5)) == "[  2:00:00:05]"
+    assert L.format_report("a%jb%", **f) == "ab"  # unknown code prints nothing; a lone % at the end too
+    assert L.format_report("%x5|", **f) == "0.18750|" and L.format_report("%xz|", **f) == "0.18750|"
+    long = "%r" * 40
+    assert L.format_report(long, **f) == "0xabcdef0123456789" * 40  # no fixed line buffer
+
+
+def test_report_table_options_from_ini(tmp_path):
+    """OutputHInterval: missing = 0 (the one-line report), negative = 15
+    (CUDALucas.cu:3459, 3488)."""
+    r = run_driver(["--print-config"], tmp_path)
+    assert "output_interval=0" in r.stdout
+    (tmp_path / "CUDALucas.ini").write_text("OutputHInterval=-1\n")
+    assert "output_interval=15" in run_driver(["--print-config"], tmp_path).stdout
+    (tmp_path / "CUDALucas.ini").write_text("OutputHInterval=19\nOutputString=%p %i\n")
+    assert "output_interval=19" in run_driver(["--print-config"], tmp_path).stdout
