@@ -2,7 +2,9 @@
 
 tests/fuzz_host_text.cpp drives ll_work_parse_line, ll_work_next,
 ll_work_clear, ll_format_report and ll_fopen_locked with seeded random
-lines, worktodo files and format strings; it is built here with g++ and
+lines, worktodo files and format strings, and the digit codec and
+checkpoint files through round trips on random small states (each round
+trip must give its input back); it is built here with g++ and
 -fsanitize=address,undefined together with cudalucas_amd/csrc/ll_host.cpp
 alone (host code only, no HIP), and any sanitizer report fails the run.
 """
