@@ -865,7 +865,48 @@ static int cufftbench(const Config &cfg, int s, int e, int passes) {
 // W x H x R (and the LDS-resident kernel) of each length is timed and the
 // fastest appended to "<device> schedules.txt".  mode & 1: only lengths in
 // the FFT-length list; mode & 8: print only the best of each length.
+// kernel_percentages (CUDALucas.cu:2002-2141), for this engine's kernels:
+// "-threadbench 0 K passes" times, in two rounds of `passes` x 50
+// iterations at length K, a whole iteration (hipGraph replay) and each
+// kernel launched on its own, and prints each kernel's share of their sum.
+static int kernel_percentages(const Config &cfg, int k, int passes) {
+  const uint32_t n = (uint32_t)k * 1024u;
+  if (k < 1 || passes < 1 || !ll_fft_supported(n)) {
+    fprintf(stderr, "can't parse -threadbench options (fft %dK, %d passes)\n\n", k, passes);
+    return 2;
+  }
+  const uint32_t p = max_exponent(n) | 1u;
+  Engine eng;
+  if (eng.open(cfg, p, n)) return 2;
+  ll_info info;
+  ll_get_info(eng.ctx, &info);
+  std::vector<int32_t> d(n);
+  ll_initial_digits(p, n, 0, d.data());
+  if (ll_load_digits(eng.ctx, d.data(), 0) || ll_iterate(eng.ctx, 100, nullptr, nullptr)) return 2;
+  double total[2] = {0, 0}, first[2] = {0, 0}, second[2] = {0, 0};
+  for (int round = 0; round < 2; round++)
+    for (int pass = 0; pass < passes; pass++) {
+      float km[2] = {0, 0};
+      if (ll_profile_kernels(eng.ctx, 50, km) || ll_iterate(eng.ctx, 50, nullptr, nullptr)) return 2;
+      first[round] += km[0];
+      second[round] += km[1];
+      total[round] += ll_last_iterate_ms(eng.ctx) / 50.0;
+    }
+  const double sum = first[0] + second[0];
+  printf("Kernel bench, fft = %dK, p = %u, %s %u x %u x %u, %d passes of 50 iterations, two rounds (ms).\n", k, p,
+         info.path ? "sweeps" : "resident", info.width, info.height, info.lines, passes);
+  printf("Total time: %11.5f, %8.5f, %6.2f\n", total[0] / passes, total[1] / passes, 100.0);
+  if (info.path) {
+    printf("Square sweep: %9.5f, %8.5f, %6.2f\n", first[0] / passes, first[1] / passes, first[0] / sum * 100.0);
+    printf("Carry sweep: %10.5f, %8.5f, %6.2f\n", second[0] / passes, second[1] / passes, second[0] / sum * 100.0);
+  } else {
+    printf("Resident kernel: %6.5f, %8.5f, %6.2f\n", first[0] / passes, first[1] / passes, 100.0);
+  }
+  return 0;
+}
+
 static int threadbench(const Config &cfg, int s, int e, int passes, int mode) {
+  if (s == 0) return kernel_percentages(cfg, e, passes);
   if (s < 1 || e < s || passes < 1) {
     fprintf(stderr, "can't parse -threadbench options\n\n");
     return 2;

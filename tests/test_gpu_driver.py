@@ -213,6 +213,24 @@ def test_cufftbench_and_threadbench_write_tuning_files(tmp_path):
     assert rep and int(rep[0][1], 16) == gmp_res64(p, int(rep[0][0]))
 
 
+def test_kernel_bench_shares(tmp_path):
+    """-threadbench 0 K passes (kernel_percentages, CUDALucas.cu:2002-2141):
+    per-iteration time and each kernel's share, for the two sweeps at 4096K
+    and the LDS-resident kernel at 4K."""
+    r = drive(["-threadbench", "0", "4096", "1", "0"], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    rows = {}
+    for line in r.stdout.splitlines():
+        name, _, rest = line.partition(":")
+        if name in ("Total time", "Square sweep", "Carry sweep"):
+            rows[name] = [float(x) for x in rest.split(",")]
+    assert set(rows) == {"Total time", "Square sweep", "Carry sweep"}, r.stdout
+    assert abs(rows["Square sweep"][2] + rows["Carry sweep"][2] - 100.0) < 0.05
+    assert all(v > 0 for row in rows.values() for v in row[:2])
+    r = drive(["-threadbench", "0", "4", "1"], tmp_path)
+    assert r.returncode == 0 and "Resident kernel:" in r.stdout, r.stdout + r.stderr
+
+
 def test_info_prints_device(tmp_path):
     r = drive(["-info"], tmp_path)  # no exponent and no worktodo: exits 2 after the listing
     assert "Device Name" in r.stdout and "multiProcessorCount" in r.stdout, r.stdout + r.stderr
