@@ -92,6 +92,7 @@ def parse_ref():
         L.isprime.argtypes = [ctypes.c_uint]
         L.IniGetInt.argtypes = [cp, cp, ctypes.POINTER(i32), i32]
         L.IniGetStr.argtypes = [cp, cp, P, cp]
+        L.IniGetInt2.argtypes = [cp, cp, ctypes.POINTER(i32), ctypes.POINTER(i32), i32]
         L.fopen_and_lock.argtypes = [cp, cp]
         L.fopen_and_lock.restype = P
         L.unlock_and_fclose.argtypes = [P]

@@ -291,9 +291,11 @@ def test_driver_ini_reading_matches_reference_readers(tmp_path):
     for case in range(60):
         lines = []
         for _ in range(rng.randrange(0, 14)):
-            key = rng.choice(list(ints) + list(strs) + ["DeviceNumbers", "workfile", " WorkFile", "Polite"])
+            key = rng.choice(list(ints) + list(strs) + ["DeviceNumbers", "workfile", " WorkFile", "Polite", "Threads"])
             if key in strs or key.lower() == "workfile":
                 val = rng.choice(["", "w%d.txt" % rng.randrange(100), "dir/r.txt", "x"])
+            elif key == "Threads":
+                val = rng.choice(["256 128", "512", "64 abc", "", "1024 256", " 128  64"])
             else:
                 val = rng.choice(["", str(rng.randrange(0, 200)), "abc", " 7", "12abc", "0x10", "-0"])
             lines.append("%s%s%s%s" % (key, rng.choice(["=", "=", " ="]), val, rng.choice(["\n", "\r\n"])))
@@ -310,6 +312,14 @@ def test_driver_ini_reading_matches_reference_readers(tmp_path):
             buf = ctypes.create_string_buffer(192)
             R.IniGetStr(os.fsencode(ini), key.encode(), buf, dflt.encode())
             assert cfg[name] == buf.value.decode(), (case, key, lines)
+        # Threads (IniGetInt2): the same pair where the reference finds one;
+        # without one this build keeps its automatic choice (0,0) where the
+        # reference falls back to 256
+        t0, t1 = ctypes.c_int(0), ctypes.c_int(0)
+        if R.IniGetInt2(os.fsencode(ini), b"Threads", ctypes.byref(t0), ctypes.byref(t1), 256):
+            assert cfg["threads"] == "%d,%d" % (t0.value, t1.value), (case, lines)
+        else:
+            assert cfg["threads"] == "0,0", (case, lines)
 
 
 def test_locked_files_wait_time_out_and_exclude_the_reference(tmp_path, capfd):
