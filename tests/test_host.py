@@ -414,3 +414,38 @@ def test_report_table_options_from_ini(tmp_path):
     assert "output_interval=15" in run_driver(["--print-config"], tmp_path).stdout
     (tmp_path / "CUDALucas.ini").write_text("OutputHInterval=19\nOutputString=%p %i\n")
     assert "output_interval=19" in run_driver(["--print-config"], tmp_path).stdout
+
+
+def test_c_abi_header_builds_as_c99_and_links(tmp_path):
+    """include/ll_api.h is plain C: a C99 client (the language of the
+    reference's host code) compiles with -pedantic -Werror, links against the
+    library and calls host-only entry points (no GPU needed)."""
+    import shutil
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    src = tmp_path / "client.c"
+    src.write_text(r'''
+#include <stdio.h>
+#include "ll_api.h"
+int main(void) {
+  ll_assignment a;
+  ll_report r = {0};
+  char line[64];
+  int st = ll_work_parse_line("Test=N/A,77005277,4M", &a);
+  r.p = 77005277; r.n = 4194304; r.iteration = 10000; r.residue = 0x1234u;
+  ll_format_report("%P %i %f %r", &r, line, sizeof line);
+  printf("%d %u %u %s|%d %d|%s\n", st, a.exponent, a.fft_length, a.key, ll_is_prime(77005277),
+         ll_fft_from_str("4096K"), line);
+  return 0;
+}
+''')
+    lib_dir = os.path.dirname(L.LIB_PATH)
+    inc = os.path.join(os.path.dirname(lib_dir), "include")
+    exe = tmp_path / "client"
+    b = subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-pedantic", "-Werror", "-I", inc, "-o", str(exe),
+                        str(src), "-L", lib_dir, "-lcudalucas_amd", "-Wl,-rpath," + lib_dir],
+                       capture_output=True, text=True, timeout=120)
+    assert b.returncode == 0, b.stderr
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.strip() == "0 77005277 4194304 N/A|1 4194304| M77005277     10000  4096K 0x0000000000001234"
