@@ -202,7 +202,7 @@ def random_workfile(rng, n_lines):
             tail = rng.choice(["", ",69,0", ",75,1", ",4096K,74,1", ",2M"])
             lines.append("%s=%s%d%s" % (rng.choice(["Test", "DoubleCheck"]), key, p, tail))
     lines = [l for l in lines if not slash_in_last_field_before_comment(l)]
-    return "".join(l + "\n" for l in lines)
+    return "".join(l + ("\r\n" if rng.random() < 0.2 else "\n") for l in lines)
 
 
 @needs_ref
@@ -211,7 +211,7 @@ def test_next_assignment_matches_reference(tmp_path):
     for case in range(150):
         text = random_workfile(rng, rng.randrange(0, 12))
         path = tmp_path / "worktodo.txt"
-        path.write_text(text)
+        path.write_text(text, newline="")
         q, n = ctypes.c_int(0), ctypes.c_int(0)
         key = ctypes.create_string_buffer(192)
         rc_ref = REF.get_next_assignment(os.fsencode(path), ctypes.byref(q), ctypes.byref(n),
@@ -238,8 +238,8 @@ def test_clear_assignment_matches_reference(tmp_path, monkeypatch):
         parsed = [L.work_parse_line(l) for l in text.splitlines()]
         listed = [a.exponent for st, a in parsed if st == L.WORK_OK]
         p = rng.choice(listed) if listed and rng.random() < 0.85 else rng.choice([44497, 1257787, 12345])
-        (ref_dir / "worktodo.txt").write_text(text)
-        (our_dir / "worktodo.txt").write_text(text)
+        (ref_dir / "worktodo.txt").write_text(text, newline="")
+        (our_dir / "worktodo.txt").write_text(text, newline="")
         rc_ref = REF.clear_assignment(b"worktodo.txt", p)
         rc = L.work_clear(our_dir / "worktodo.txt", p)
         assert rc == rc_ref, (case, p, text)
