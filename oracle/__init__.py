@@ -63,6 +63,7 @@ def lib():
         L.orc_ibdwt_iterate.argtypes = [P, u64, P]
         L.orc_ibdwt_iterate.restype = dbl
         L.orc_ibdwt_read.argtypes = [P, P, P]
+        L.orc_set_threads.argtypes = [ctypes.c_int]
         _lib = L
     return _lib
 
@@ -98,6 +99,12 @@ def parse_ref():
         L.unlock_and_fclose.argtypes = [P]
         _parse_ref = L
     return _parse_ref
+
+
+def set_threads(n):
+    """Threads for the IBDWT's data-parallel loops; returns the count in
+    effect.  Results are the same for every count."""
+    return int(lib().orc_set_threads(n))
 
 
 def _ptr(a):

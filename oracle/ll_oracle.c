@@ -34,6 +34,24 @@ uint64_t orc_splitmix64(uint64_t *state) {
   return z ^ (z >> 31);
 }
 
+/* Threads for the IBDWT's data-parallel loops (FFT passes, the pair
+ * squaring, rounding and reweighting); the carry chain stays sequential.
+ * Each element is computed by the same operations whichever thread runs it,
+ * so the results do not depend on the count.  Default 1: no thread pool is
+ * started unless a caller asks (the oracle is also used from forked
+ * processes). */
+static int g_threads = 1;
+
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+  g_threads = n < 1 ? 1 : n;
+#else
+  (void)n;
+  g_threads = 1;
+#endif
+  return g_threads;
+}
+
 static uint64_t mulmod64(uint64_t a, uint64_t b, uint64_t m) {
   return (uint64_t)((unsigned __int128)a * b % m);
 }
@@ -509,6 +527,7 @@ static void fft_run(fft_plan *pl, double *data) {
     uint32_t R = (uint32_t)pl->rad[pass];
     uint32_t stride = m / R;
     uint32_t twstep = m / (ns * R);
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
     for (uint32_t j = 0; j < stride; j++) {
       uint32_t k = j % ns;
       double vr[7], vi[7];
@@ -648,6 +667,7 @@ void orc_ibdwt_load(orc_ibdwt *c, const int32_t *digits, uint32_t offset) {
  * with W = (0.5 - ct[n/4 - j]) + i ct[j]. */
 static void square_split(double *a, uint32_t n, const double *ct) {
   uint32_t m = n / 2;
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
   for (uint32_t j = 1; j < n / 4; j++) {
     double wr = 0.5 - ct[n / 4 - j], wi = ct[j];
     double ar = a[2 * j], ai = a[2 * j + 1];
@@ -698,6 +718,7 @@ double orc_ibdwt_iterate(orc_ibdwt *c, uint64_t count, double *per_iter) {
     fft_run(&c->plan, c->x);
 
     double maxerr = 0.0;
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1) reduction(max : maxerr)
     for (uint32_t j = 0; j < n; j++) {
       double t = c->x[j] * c->ttmp[j];
       double r = rint(t);
@@ -718,6 +739,7 @@ double orc_ibdwt_iterate(orc_ibdwt *c, uint64_t count, double *per_iter) {
       carry = (t - d) >> w;
       c->v[j] = d;
     }
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
     for (uint32_t j = 0; j < n; j++) c->x[j] = (double)c->v[j] * c->ttp[j];
     if (per_iter) per_iter[it] = maxerr;
     if (maxerr > maxall) maxall = maxerr;

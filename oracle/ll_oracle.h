@@ -82,6 +82,9 @@ void orc_ibdwt_load(orc_ibdwt *c, const int32_t *digits, uint32_t offset);
 /* Runs count iterations; per_iter (may be NULL) receives each iteration's
  * maximum |x - rint(x)|; returns the maximum over all of them. */
 double orc_ibdwt_iterate(orc_ibdwt *c, uint64_t count, double *per_iter);
+/* Threads for the IBDWT's data-parallel loops (1 by default); returns the
+ * count in effect (1 without OpenMP).  Results do not depend on it. */
+int orc_set_threads(int n);
 void orc_ibdwt_read(orc_ibdwt *c, int32_t *digits, uint32_t *offset);
 
 #ifdef __cplusplus

@@ -122,6 +122,27 @@ def test_ibdwt_random_state_matches_gmp():
     assert (O.digits_to_residue(p, n, d2, o2) == expect).all()
 
 
+def test_ibdwt_threads_change_nothing():
+    """The IBDWT's data-parallel loops give bit-identical digits and
+    per-iteration errors for any thread count, and still match GMP."""
+    p, n = 1257787, 65536
+    runs = []
+    for threads in (1, 4):
+        assert O.set_threads(threads) in (1, threads)
+        try:
+            c = O.IBDWT(p, n)
+            c.load(O.initial_digits(p, n, 5), 5)
+            _, errs = c.iterate(40, per_iter=True)
+            d, off = c.read()
+            c.close()
+        finally:
+            O.set_threads(1)
+        runs.append((d.copy(), off, np.asarray(errs).copy()))
+    (d1, o1, e1), (d4, o4, e4) = runs
+    assert o1 == o4 and (d1 == d4).all() and (e1 == e4).all()
+    assert (O.digits_to_residue(p, n, d4, o4) == O.gmp_ll(p, 40)).all()
+
+
 def test_planted_error_is_caught():
     """A plausible slip (forgetting the -2) must fail the GMP comparison."""
     p, n, iters = 44497, 4096, 50
