@@ -669,7 +669,11 @@ static int check_residue(const Config &cfg, int longtest) {
 static std::string device_file(int device, const char *suffix) {
   ll_device_info d;
   if (ll_get_device_info(device < 0 ? 0 : device, &d) != LL_OK) return "";
-  return std::string(d.name) + " " + suffix;
+  // Some ROCm installs report an empty device name: the architecture
+  // ("gfx950") names the files then.
+  std::string name(d.name);
+  if (name.empty()) name = std::string(d.arch).substr(0, std::string(d.arch).find(':'));
+  return name + " " + suffix;
 }
 
 // A data file shipped in data/ beside the binary.

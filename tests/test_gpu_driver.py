@@ -195,14 +195,14 @@ def test_cufftbench_and_threadbench_write_tuning_files(tmp_path):
     r = drive(["-cufftbench", 16, 40, 1], tmp_path)
     assert r.returncode == 0, r.stdout + r.stderr
     fft = [f for f in os.listdir(tmp_path) if f.endswith(" fft.txt")]
-    assert len(fft) == 1
+    assert len(fft) == 1 and not fft[0].startswith(" ")  # named for the device (or its architecture)
     rows = [l.split() for l in (tmp_path / fft[0]).read_text().splitlines() if l[:5].strip().isdigit()]
     assert rows and all(L.fft_supported(int(k) * 1024) and float(ms) > 0 for k, _, ms in rows)
     assert "16" in [k for k, _, _ in rows]  # powers of two are always kept
     r = drive(["-threadbench", 256, 256, 1, 8], tmp_path)
     assert r.returncode == 0, r.stdout + r.stderr
     sch = [f for f in os.listdir(tmp_path) if f.endswith(" schedules.txt")]
-    assert len(sch) == 1
+    assert len(sch) == 1 and not sch[0].startswith(" ")
     k, path, w, h, rr, ms = (tmp_path / sch[0]).read_text().split()[:6]
     assert (int(k), int(path)) == (256, 1) and int(w) * int(h) * 2 == 262144
     # the files are read back: a run still matches GMP
