@@ -969,6 +969,7 @@ static void usage() {
           "        save the fastest ones in \"<device name> fft.txt\"\n"
           "  -threadbench start end passes [mode]: time every kernel schedule of each length and\n"
           "        append the fastest to \"<device name> schedules.txt\"\n"
+          "  -threadbench 0 fft passes: time an iteration of length fft K and each kernel's share\n"
           "  --selector model [--selector-times file]: learned fft length selection\n"
           "  -polite and -k are accepted for compatibility and have no effect.\n");
 }

@@ -269,6 +269,7 @@ def test_tuner_table_round_trip(tmp_path):
 def test_driver_bench_options_parse(tmp_path):
     r = run_driver(["-h"], tmp_path)
     assert "-cufftbench" in r.stderr and "-threadbench" in r.stderr and "--selector" in r.stderr
+    assert "-threadbench 0 fft passes" in r.stderr
     r = run_driver(["-cufftbench", "1"], tmp_path)
     assert r.returncode == 2 and "can't parse -cufftbench" in r.stderr
     r = run_driver(["-memtest", "10", "1"], tmp_path)
