@@ -386,8 +386,20 @@ __global__ __launch_bounds__(256) void line_pass_table(const double *qc, cplx *o
   build_wave_table<W, 256, auto_plan_uniform<W, R, T>()>(out, QuarterRoots<W>{qc});
 }
 
+// Workgroups of carry_sweep<W, R, T> that the LDS of a CU holds, at most 2:
+// the launch bound asks for no more occupancy than that, so a sweep whose
+// lines fill over half the LDS is not held to half the register file.
 template <int W, int R, int T>
-__global__ __launch_bounds__(T, T >= 512 ? 1 : 2) void carry_sweep(const PlanArgs *__restrict__ args) {
+constexpr int carry_groups_per_cu() {
+  constexpr bool tab = carry_pass_table<W, R, T>();
+  constexpr int bytes = R * lpitch(W) * 16 + (tab ? 8 : (W / 4 + 1) * 8) + 2 * R * 4 + 4 * R * 8 + (T / 64) * 4 +
+                        (tab ? 16 * W : 16);
+  return 163840 / bytes >= 2 ? 2 : 1;
+}
+
+template <int W, int R, int T>
+__global__ __launch_bounds__(T, (T >= 512 ? 1 : carry_groups_per_cu<W, R, T>())) void carry_sweep(
+    const PlanArgs *__restrict__ args) {
   const PlanArgs &A = args[blockIdx.y];
   constexpr int LP = lpitch(W);
   constexpr int CPT = (W + T - 1) / T;  // columns per thread
