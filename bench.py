@@ -63,7 +63,8 @@ def cpu_baseline(p, n, budget_s):
     import cudalucas_amd as L
     import oracle as O
 
-    want = int(os.environ.get("OMP_NUM_THREADS") or 0) or len(os.sched_getaffinity(0))
+    omp = (os.environ.get("OMP_NUM_THREADS") or "").split(",")[0].strip()  # "16" or a list "16,8"
+    want = (int(omp) if omp.isdigit() else 0) or len(os.sched_getaffinity(0))
     legs = []
     for threads in ([want, 1] if want > 1 else [1]):
         used = O.set_threads(threads)
