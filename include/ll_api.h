@@ -342,7 +342,8 @@ typedef struct ll_assignment {
 #define LL_WORK_BLANK 6
 #define LL_WORK_COMMENT 7        /* a line starting with #, // or \\ */
 /* parse_worktodo_line (parse.c:241-484) on one line; a trailing newline is
- * ignored. Fills out (zeroed first) and returns an LL_WORK_* status. */
+ * ignored. Fills out (zeroed first) and returns an LL_WORK_* status
+ * (LL_ERR_ARG if out is NULL). */
 int ll_work_parse_line(const char *line, ll_assignment *out);
 /* isprime (parse.c:155-174): 1 if n is prime. */
 int ll_is_prime(uint32_t n);
@@ -352,7 +353,8 @@ int ll_is_prime(uint32_t n);
 int ll_valid_assignment(uint32_t p, uint32_t fft_length);
 /* get_next_assignment (parse.c:486-554): the first line that parses and is
  * valid; each other non-blank, non-comment line before it is reported on
- * stderr and skipped. Returns 0, 1 = cannot open the file, 2 = none found. */
+ * stderr and skipped. Returns 0, 1 = cannot open the file, 2 = none found
+ * (LL_ERR_ARG for a NULL argument). */
 int ll_work_next(const char *file, ll_assignment *out);
 /* clear_assignment (parse.c:556-678): rewrites the file without the first
  * assignment for exponent p.  The comment, blank and unparsable lines just
@@ -360,7 +362,7 @@ int ll_work_next(const char *file, ll_assignment *out);
  * above, back to the file's first blank line (lines before that stay).
  * Returns 0, 3 = cannot open the file, 4 = cannot create the temporary
  * "<file>.tmp", 5 = no such assignment (file unchanged), 6 = cannot replace
- * the file. */
+ * the file (LL_ERR_ARG for a NULL file). */
 int ll_work_clear(const char *file, uint32_t p);
 
 /* ---- the report line (process_output + encode_output_options,
@@ -383,7 +385,8 @@ typedef struct ll_report {
  * a digit k = 0..5 (roundoff, ms/iteration, interval seconds, iterations/s,
  * percent done, printed with 5 decimals and cut to k).  Other text is
  * copied.  Writes at most cap bytes (NUL-terminated) and returns the length
- * of the whole line, as snprintf does. */
+ * of the whole line, as snprintf does (LL_ERR_ARG for a NULL format or r;
+ * out may be NULL with cap 0 to ask for the length). */
 int ll_format_report(const char *format, const ll_report *r, char *out, size_t cap);
 
 /* ---- locked files (fopen_and_lock / unlock_and_fclose, parse.c:775-894) - */
