@@ -150,6 +150,58 @@ def batched_leg(steps=4, step_iters=512, plain_iters=200):
     }
 
 
+def smooth_leg(iters=2000, warmup=100):
+    """The smooth-length sweep: each 3/5/7-smooth length with its seeded
+    exponent at ~18.5 bits/word, ms/iteration over `iters` hipGraph-replayed
+    iterations (wall clock and hipEvent GPU time) and HBM GB/s at the
+    algorithmic 32 N bytes per iteration."""
+    import cudalucas_amd as L
+    from cudalucas_amd import workloads as W
+
+    rows = []
+    for n in W.SMOOTH_LENGTHS:
+        p = W.smooth_prime(n)
+        with L.LLContext(p, n) as c:
+            info = c.info()
+            c.load_initial(0)
+            c.iterate(warmup)
+            t0 = time.perf_counter()
+            m = c.iterate(iters)
+            wall = time.perf_counter() - t0
+            gpu = c.last_iterate_ms() / iters
+        b_alg, _ = algorithmic(n)
+        rows.append({
+            "n": n, "p": p, "bits_per_word": round(p / n, 3),
+            "schedule": "W=%d H=%d R=%d" % (info["width"], info["height"], info["lines"]),
+            "ms_per_iter": round(wall * 1000.0 / iters, 5), "gpu_ms_per_iter": round(gpu, 5),
+            "hbm_gbs_alg": round(b_alg / (gpu * 1e-3) / 1e9, 1),
+            "hbm_frac_alg": round(b_alg / (gpu * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "max_roundoff": round(float(m), 4),
+        })
+    return rows
+
+
+def small_leg():
+    """The plain single exponent at N = 4096 (LDS-resident): all p - 2
+    iterations in one call, microseconds per iteration."""
+    import cudalucas_amd as L
+    from cudalucas_amd import workloads as W
+
+    p, n = W.SMALL_P, W.SMALL_N
+    with L.LLContext(p, n) as c:
+        c.load_initial(0)
+        c.iterate(1000)
+        c.load_initial(0)
+        t0 = time.perf_counter()
+        m = c.iterate(p - 2)
+        wall = time.perf_counter() - t0
+        gpu = c.last_iterate_ms() / (p - 2)
+        _, r = c.residue()
+    return {"p": p, "n": n, "iterations": p - 2, "us_per_iter": round(wall * 1e6 / (p - 2), 3),
+            "gpu_us_per_iter": round(gpu * 1e3, 3), "iter_per_s": round((p - 2) / wall, 1),
+            "max_roundoff": round(float(m), 4), "res64": "0x%016x" % r}
+
+
 def load_traffic(name):
     path = os.path.join(HERE, "profiles", name)
     if not os.path.exists(path):
@@ -169,6 +221,7 @@ def main():
     ap.add_argument("--copy-only", action="store_true",
                     help="only the 64 MiB streaming-copy calibration kernel")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-smooth", action="store_true", help="skip the smooth-length and N=4096 legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: measures one GPU; the LL path of one exponent runs on a single "
@@ -256,6 +309,15 @@ def main():
         "res64_iteration": done_iters,
         "roofline": roofline,
     }
+    if not args.no_smooth:
+        if single_tuned:
+            T.install(T.load(T.SINGLE_TABLE))
+        try:
+            out["smooth"] = smooth_leg()
+        finally:
+            if single_tuned:
+                T.install([])
+        out["small"] = small_leg()
     if not args.no_batched:
         out["batched"] = batched_leg()
     if not args.no_cpu:

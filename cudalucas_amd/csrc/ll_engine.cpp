@@ -108,6 +108,13 @@ bool pick_schedule_auto(uint32_t n, const ll_options *o, Schedule *s) {
 
 }  // namespace
 
+// Small host values that cross to and from the device on every call.
+struct HostWords {
+  uint32_t offset;
+  uint32_t unload_err;
+  ResidentDesc rdesc;
+};
+
 struct ll_ctx {
   uint32_t p = 0, n = 0, m = 0;
   int device = 0;
@@ -142,7 +149,10 @@ struct ll_ctx {
   ResidentDesc *d_rdesc = nullptr;
   float *r_err = nullptr;
   uint32_t *r_offset = nullptr;  // device shift offset of the resident path
-  std::vector<uint32_t> host_ring;
+  // pinned host memory: the error slots of one call, the offset, the
+  // resident descriptor (so readbacks and uploads need no staging)
+  uint32_t *h_ring = nullptr;
+  HostWords *h_words = nullptr;
   // timing (hipEvents on `stream`)
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   double last_gpu_ms = 0.0;
@@ -253,7 +263,6 @@ static int build_sweeps(ll_ctx *c) {
   A.q = p / n;
   A.b = p % n;
   A.remaining = nullptr;
-  c->host_ring.resize(c->ring_size);
   if ((rc = dalloc(c, &c->d_args, 1))) return rc;
   if ((rc = dalloc(c, &c->d_args_unload, 1))) return rc;
   HIPCHK(c, hipMemcpy(c->d_args, &A, sizeof(PlanArgs), hipMemcpyHostToDevice));
@@ -284,7 +293,6 @@ static int build_resident(ll_ctx *c) {
   D.p = p;
   D.m = M;
   D.iters = 0;
-  c->host_ring.resize(c->ring_size);
   return LL_OK;
 }
 
@@ -295,6 +303,10 @@ static void release(ll_ctx *c) {
   c->graph = nullptr;
   for (void *p : c->allocs) (void)hipFree(p);
   c->allocs.clear();
+  if (c->h_ring) (void)hipHostFree(c->h_ring);
+  if (c->h_words) (void)hipHostFree(c->h_words);
+  c->h_ring = nullptr;
+  c->h_words = nullptr;
   if (c->stream) (void)hipStreamDestroy(c->stream);
   c->stream = nullptr;
 }
@@ -324,15 +336,19 @@ int ll_fft_supported(uint32_t n) {
   return pick_schedule(n, nullptr, &s) ? 1 : 0;
 }
 
-static int ctx_create(uint32_t p, uint32_t n, const ll_options *opts, uint32_t ring_size, ll_ctx **out);
+static int ctx_create(uint32_t p, uint32_t n, const ll_options *opts, uint32_t ring_size, bool graph,
+                      ll_ctx **out);
 
 int ll_create(uint32_t p, uint32_t n, const ll_options *opts, ll_ctx **out) {
-  return ctx_create(p, n, opts, kRingSize, out);
+  return ctx_create(p, n, opts, kRingSize, true, out);
 }
 
 }  // extern "C"
 
-static int ctx_create(uint32_t p, uint32_t n, const ll_options *opts, uint32_t ring_size, ll_ctx **out) {
+// graph: instantiate the iteration graph now (two-sweep path), so that no
+// ll_iterate call pays for it.
+static int ctx_create(uint32_t p, uint32_t n, const ll_options *opts, uint32_t ring_size, bool graph,
+                      ll_ctx **out) {
   if (!out) return LL_ERR_ARG;
   *out = nullptr;
   if (p < 3 || n < 512 || n % 2 || (uint64_t)p < 4ull * n / 3) return LL_ERR_ARG;
@@ -361,10 +377,14 @@ static int ctx_create(uint32_t p, uint32_t n, const ll_options *opts, uint32_t r
     e = hipEventCreate(&c->ev[i]);
     if (e != hipSuccess) rc = fail(c, LL_ERR_HIP, "hipEventCreate: %s", hipGetErrorString(e));
   }
+  if (!rc && (hipHostMalloc((void **)&c->h_ring, sizeof(uint32_t) * ring_size, 0) != hipSuccess ||
+              hipHostMalloc((void **)&c->h_words, sizeof(HostWords), 0) != hipSuccess))
+    rc = fail(c, LL_ERR_ALLOC, "hipHostMalloc of the readback buffers failed");
   if (!rc) rc = dalloc(c, &c->x, c->m);
   if (!rc) rc = dalloc(c, &c->digits, n);
   if (!rc) rc = dalloc(c, &c->unload_err, 1);
   if (!rc) rc = c->sch.path == 0 ? build_resident(c) : build_sweeps(c);
+  if (!rc && graph && c->sch.path == 1 && c->use_graph) rc = build_graph(c);
   *out = c;
   return rc;
 }
@@ -384,14 +404,19 @@ int ll_load_digits(ll_ctx *c, const int32_t *digits, uint32_t offset) {
   HIPCHK(c, hipSetDevice(c->device));
   offset %= c->p;
   HIPCHK(c, hipMemcpyAsync(c->digits, digits, sizeof(int32_t) * c->n, hipMemcpyHostToDevice, c->stream));
+  c->h_words->offset = offset;
   if (c->sch.path == 0) {
-    c->rdesc.iters = 0;
-    HIPCHK(c, hipMemcpyAsync(c->d_rdesc, &c->rdesc, sizeof(ResidentDesc), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->r_offset, &offset, sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+    c->h_words->rdesc = c->rdesc;
+    c->h_words->rdesc.iters = 0;
+    HIPCHK(c, hipMemcpyAsync(c->d_rdesc, &c->h_words->rdesc, sizeof(ResidentDesc), hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->r_offset, &c->h_words->offset, sizeof(uint32_t), hipMemcpyHostToDevice,
+                             c->stream));
     HIPCHK(c, launch_resident_load(c->d_rdesc, 1, c->digits, c->stream));
   } else {
     HIPCHK(c, launch_load(c->sch.ck, c->d_args, 1, c->sch.H, c->stream));
-    HIPCHK(c, hipMemcpyAsync(&c->st->offset, &offset, sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&c->st->offset, &c->h_words->offset, sizeof(uint32_t), hipMemcpyHostToDevice,
+                             c->stream));
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->offset = offset;
@@ -400,26 +425,29 @@ int ll_load_digits(ll_ctx *c, const int32_t *digits, uint32_t offset) {
   return LL_OK;
 }
 
+// Both iterate forms run `count` <= ring_size iterations, then bring back
+// only the error slots those iterations wrote, and the offset, through
+// pinned host memory: a call costs its kernels plus one synchronisation.
 static int iterate_resident(ll_ctx *c, uint32_t count, float *errs) {
-  ResidentDesc D = c->rdesc;
-  D.iters = count;
-  HIPCHK(c, hipMemcpyAsync(c->d_rdesc, &D, sizeof(ResidentDesc), hipMemcpyHostToDevice, c->stream));
+  HostWords &hw = *c->h_words;
+  hw.rdesc = c->rdesc;
+  hw.rdesc.iters = count;
+  HIPCHK(c, hipMemcpyAsync(c->d_rdesc, &hw.rdesc, sizeof(ResidentDesc), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
   HIPCHK(c, launch_resident(c->sch.rk, c->d_rdesc, 1, c->stream));
   HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-  HIPCHK(c, hipMemcpyAsync(errs, c->r_err, sizeof(float) * count, hipMemcpyDeviceToHost, c->stream));
-  uint32_t off = 0;
-  HIPCHK(c, hipMemcpyAsync(&off, c->r_offset, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->h_ring, c->r_err, sizeof(float) * count, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(&hw.offset, c->r_offset, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   float ms = 0.f;
   HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
   c->last_gpu_ms += ms;
-  c->offset = off;
+  memcpy(errs, c->h_ring, sizeof(float) * count);
+  c->offset = hw.offset;
   return LL_OK;
 }
 
 static int iterate_sweeps(ll_ctx *c, uint32_t count, float *errs) {
-  HIPCHK(c, launch_fill(c->ring, 0u, c->ring_size, c->stream));
   HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
   uint32_t left = count;
   if (c->use_graph && count >= kGraphIters) {
@@ -435,20 +463,22 @@ static int iterate_sweeps(ll_ctx *c, uint32_t count, float *errs) {
     HIPCHK(c, launch_carry(c->sch.ck, c->d_args, 1, c->sch.H, c->stream));
   }
   HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-  HIPCHK(c, hipMemcpyAsync(c->host_ring.data(), c->ring, sizeof(uint32_t) * c->ring_size,
-                           hipMemcpyDeviceToHost, c->stream));
-  uint32_t off = 0;
-  HIPCHK(c, hipMemcpyAsync(&off, &c->st->offset, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+  // the slots of epochs epoch .. epoch + count - 1, which may wrap the ring
+  const uint32_t first = (uint32_t)(c->epoch & (c->ring_size - 1));
+  const uint32_t n1 = std::min(count, c->ring_size - first);
+  HIPCHK(c, hipMemcpyAsync(c->h_ring, c->ring + first, sizeof(uint32_t) * n1, hipMemcpyDeviceToHost, c->stream));
+  if (n1 < count)
+    HIPCHK(c, hipMemcpyAsync(c->h_ring + n1, c->ring, sizeof(uint32_t) * (count - n1), hipMemcpyDeviceToHost,
+                             c->stream));
+  HIPCHK(c, hipMemcpyAsync(&c->h_words->offset, &c->st->offset, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                           c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   float ms = 0.f;
   HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
   c->last_gpu_ms += ms;
-  for (uint32_t i = 0; i < count; i++) {
-    const uint32_t bits = c->host_ring[(c->epoch + i) & (c->ring_size - 1)];
-    memcpy(&errs[i], &bits, 4);
-  }
+  memcpy(errs, c->h_ring, sizeof(float) * count);
   c->epoch += count;
-  c->offset = off;
+  c->offset = c->h_words->offset;
   return LL_OK;
 }
 
@@ -491,23 +521,23 @@ int ll_read_digits(ll_ctx *c, int32_t *digits, uint32_t *offset) {
   if (!c->loaded) return fail(c, LL_ERR_STATE, "ll_read_digits before ll_load_digits");
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, launch_fill(c->unload_err, 0u, 1, c->stream));
-  uint32_t dev_offset = 0;
+  HostWords &hw = *c->h_words;
   if (c->sch.path == 0) {
-    ResidentDesc D = c->rdesc;
-    D.iters = 0;
-    HIPCHK(c, hipMemcpyAsync(c->d_rdesc, &D, sizeof(ResidentDesc), hipMemcpyHostToDevice, c->stream));
+    hw.rdesc = c->rdesc;
+    hw.rdesc.iters = 0;
+    HIPCHK(c, hipMemcpyAsync(c->d_rdesc, &hw.rdesc, sizeof(ResidentDesc), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, launch_resident_unload(c->d_rdesc, 1, c->digits, c->unload_err, c->stream));
-    HIPCHK(c, hipMemcpyAsync(&dev_offset, c->r_offset, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&hw.offset, c->r_offset, 4, hipMemcpyDeviceToHost, c->stream));
   } else {
     HIPCHK(c, launch_unload(c->sch.ck, c->d_args_unload, 1, c->sch.H, c->stream));
-    HIPCHK(c, hipMemcpyAsync(&dev_offset, &c->st->offset, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&hw.offset, &c->st->offset, 4, hipMemcpyDeviceToHost, c->stream));
   }
-  uint32_t ebits = 0;
   HIPCHK(c, hipMemcpyAsync(digits, c->digits, sizeof(int32_t) * c->n, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipMemcpyAsync(&ebits, c->unload_err, 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(&hw.unload_err, c->unload_err, 4, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  const uint32_t dev_offset = hw.offset;
   float e;
-  memcpy(&e, &ebits, 4);
+  memcpy(&e, &hw.unload_err, 4);
   if (!(e < 0.05f)) return fail(c, LL_ERR_ROUNDOFF, "digit readback rounding error %g", (double)e);
   c->offset = dev_offset;
   if (offset) *offset = dev_offset;
@@ -535,7 +565,6 @@ int ll_profile_kernels(ll_ctx *c, uint32_t count, float *kernel_ms) {
     c->iterations += count;
     return LL_OK;
   }
-  HIPCHK(c, launch_fill(c->ring, 0u, c->ring_size, c->stream));
   double sq = 0.0, ca = 0.0;
   for (uint32_t i = 0; i < count; i++) {
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
@@ -655,6 +684,9 @@ struct ll_batch {
   std::vector<BatchGroup> groups;
   std::vector<void *> allocs;
   uint32_t *d_remaining = nullptr;
+  uint32_t *d_ring = nullptr;        // per test kBatchRing error slots
+  uint32_t *h_ring = nullptr;        // pinned copy of d_ring
+  uint32_t *h_remaining = nullptr;   // pinned copy of d_remaining
   std::vector<uint32_t> remaining;   // host copy after the last step
   std::vector<uint64_t> done;        // iterations run per test
   std::vector<float> max_err;        // running maximum per test
@@ -684,6 +716,9 @@ static void batch_release(ll_batch *b) {
   b->tests.clear();
   for (void *p : b->allocs) (void)hipFree(p);
   b->allocs.clear();
+  if (b->h_ring) (void)hipHostFree(b->h_ring);
+  if (b->h_remaining) (void)hipHostFree(b->h_remaining);
+  b->h_ring = b->h_remaining = nullptr;
   for (auto &e : b->ev)
     if (e) (void)hipEventDestroy(e);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -748,7 +783,11 @@ int ll_batch_create(int32_t device, uint32_t count, const ll_batch_test *tests, 
   BCHK(b, hipStreamCreateWithFlags(&b->stream2, hipStreamNonBlocking));
   for (auto &e : b->ev) BCHK(b, hipEventCreate(&e));
   int rc = balloc(b, &b->d_remaining, count);
+  if (!rc) rc = balloc(b, &b->d_ring, (size_t)count * kBatchRing);
   if (rc) return rc;
+  if (hipHostMalloc((void **)&b->h_ring, sizeof(uint32_t) * count * kBatchRing, 0) != hipSuccess ||
+      hipHostMalloc((void **)&b->h_remaining, sizeof(uint32_t) * count, 0) != hipSuccess)
+    return bfail(b, LL_ERR_ALLOC, "hipHostMalloc of the batch readback buffers failed");
   b->remaining.resize(count);
   b->done.assign(count, 0);
   b->max_err.assign(count, 0.f);
@@ -758,7 +797,7 @@ int ll_batch_create(int32_t device, uint32_t count, const ll_batch_test *tests, 
   o.flags = flags & LL_OPT_FORCE_SWEEPS;
   for (uint32_t i = 0; i < count; i++) {
     ll_ctx *c = nullptr;
-    rc = ctx_create(tests[i].p, tests[i].n, &o, kBatchRing, &c);
+    rc = ctx_create(tests[i].p, tests[i].n, &o, kBatchRing, false, &c);
     if (c) b->tests.push_back(c);
     if (rc) return bfail(b, rc, "test %u (p=%u n=%u): %s", i, tests[i].p, tests[i].n, c ? c->msg : "");
     std::vector<int32_t> d(tests[i].n);
@@ -787,6 +826,8 @@ int ll_batch_create(int32_t device, uint32_t count, const ll_batch_test *tests, 
       for (size_t t = 0; t < k; t++) {
         a[t] = b->tests[g.members[t]]->args;
         a[t].remaining = b->d_remaining + g.members[t];
+        a[t].err = b->d_ring + (size_t)g.members[t] * kBatchRing;
+        a[t].err_mask = kBatchRing - 1;
       }
       if ((rc = balloc(b, &g.d_args, k))) return rc;
       BCHK(b, hipMemcpy(g.d_args, a.data(), k * sizeof(PlanArgs), hipMemcpyHostToDevice));
@@ -795,6 +836,7 @@ int ll_batch_create(int32_t device, uint32_t count, const ll_batch_test *tests, 
       for (size_t t = 0; t < k; t++) {
         g.desc[t] = b->tests[g.members[t]]->rdesc;
         g.desc[t].remaining = b->d_remaining + g.members[t];
+        g.desc[t].err = (float *)(b->d_ring + (size_t)g.members[t] * kBatchRing);
       }
       if ((rc = balloc(b, &g.d_desc, k))) return rc;
     }
@@ -818,11 +860,8 @@ int ll_batch_step(ll_batch *b, uint32_t iterations, float *step_max_err) {
   BCHK(b, hipSetDevice(b->device));
   const uint32_t count = (uint32_t)b->tests.size();
   std::vector<uint32_t> before = b->remaining;
-  // clear the error rings of the tests
-  for (auto *c : b->tests) {
-    uint32_t *ring = c->sch.path == 0 ? (uint32_t *)c->r_err : c->ring;
-    BCHK(b, launch_fill(ring, 0u, c->ring_size, b->stream));
-  }
+  // (error slots need no clearing: the resident kernel writes each of its
+  // iterations' slots, and the square sweep clears its iteration's slot)
   BCHK(b, hipEventRecord(b->ev[0], b->stream));
   BCHK(b, hipStreamWaitEvent(b->stream2, b->ev[0], 0));
   // LDS-resident groups: one launch for the whole step, on the second stream
@@ -851,15 +890,11 @@ int ll_batch_step(ll_batch *b, uint32_t iterations, float *step_max_err) {
   BCHK(b, hipEventRecord(b->ev[1], b->stream2));
   BCHK(b, hipStreamWaitEvent(b->stream, b->ev[1], 0));
   BCHK(b, hipEventRecord(b->ev[2], b->stream));
-  BCHK(b, hipMemcpyAsync(b->remaining.data(), b->d_remaining, 4 * count, hipMemcpyDeviceToHost, b->stream));
-  std::vector<std::vector<uint32_t>> rings(count);
-  for (uint32_t i = 0; i < count; i++) {
-    ll_ctx *c = b->tests[i];
-    rings[i].resize(c->ring_size);
-    const void *src = c->sch.path == 0 ? (const void *)c->r_err : (const void *)c->ring;
-    BCHK(b, hipMemcpyAsync(rings[i].data(), src, 4 * c->ring_size, hipMemcpyDeviceToHost, b->stream));
-  }
+  BCHK(b, hipMemcpyAsync(b->h_remaining, b->d_remaining, 4 * count, hipMemcpyDeviceToHost, b->stream));
+  BCHK(b, hipMemcpyAsync(b->h_ring, b->d_ring, sizeof(uint32_t) * count * kBatchRing, hipMemcpyDeviceToHost,
+                         b->stream));
   BCHK(b, hipStreamSynchronize(b->stream));
+  memcpy(b->remaining.data(), b->h_remaining, 4 * count);
   float ms = 0.f;
   BCHK(b, hipEventElapsedTime(&ms, b->ev[0], b->ev[2]));
   b->last_ms = ms;
@@ -869,8 +904,9 @@ int ll_batch_step(ll_batch *b, uint32_t iterations, float *step_max_err) {
     const uint32_t ran = before[i] - b->remaining[i];
     float m = 0.f;
     b->step_errs[i].resize(ran);
+    const uint32_t *ring = b->h_ring + (size_t)i * kBatchRing;
     for (uint32_t k = 0; k < ran; k++) {
-      const uint32_t bits = c->sch.path == 0 ? rings[i][k] : rings[i][(c->epoch + k) & (c->ring_size - 1)];
+      const uint32_t bits = c->sch.path == 0 ? ring[k] : ring[(c->epoch + k) & (kBatchRing - 1)];
       float e;
       memcpy(&e, &bits, 4);
       b->step_errs[i][k] = e;

@@ -173,6 +173,9 @@ __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict_
         st.m2_digit = m2.digit;
         st.m2_value = m2.value;
         st.offset = m2.new_offset;
+        // the error slot of this iteration, which the carry sweep's
+        // workgroups raise with atomicMax after this launch has ended
+        A.err[st.epoch & A.err_mask] = 0u;
         st.epoch += 1;
         st.run = 1;
       }

@@ -93,11 +93,41 @@ LARGE_REF_RES64 = 0x626AB6B09B607BA9
 # Smooth-length sweep: p at ~18.5 bits/word for 3*2^k, 5*2^k, 7*2^k lengths.
 SMOOTH_SEED = 3003
 SMOOTH_LENGTHS = [1572864, 2621440, 3670016, 5242880, 6291456]
+SMOOTH_ITERS = 2000
+# the power-of-two length each smooth length's residue is compared with
+SMOOTH_POW2 = {1572864: 2097152, 2621440: 4194304, 3670016: 4194304, 5242880: 8388608,
+               6291456: 8388608}
 
 
 def smooth_prime(n, seed=SMOOTH_SEED, bits_per_word=18.5):
     target = int(bits_per_word * n)
     return seeded_prime(seed + n, target - 1000, target)
+
+
+# Length coverage: for each default FFT length, the densest exponent the
+# reference's policy assigns to it (the largest p whose lower bound
+# 0.999 * 1024 * 3.57822505975293e-5 * p^1.0221799779697 stays within the
+# length, CUDALucas.cu:1034-1050), capped at the supported 1.1e9
+# (CHANGELOG:12): a seeded prime in the 100,000 exponents below that.
+LENGTH_SEED = 6006
+MAX_EXPONENT = 1_100_000_000
+
+
+def densest_exponent_bound(n):
+    ff1 = 0.999 * 1024.0 * 0.0000357822505975293
+    e1 = 1.022179977969700
+    return min(int((n / ff1) ** (1.0 / e1)), MAX_EXPONENT)
+
+
+def densest_prime(n, seed=LENGTH_SEED):
+    hi = densest_exponent_bound(n) - 1000
+    return seeded_prime(seed + n, hi - 100_000, hi)
+
+
+def length_check_iters(p):
+    """GMP-pinned prefix used for the length-coverage cases: 200 iterations,
+    100 above 2e8 (GMP takes seconds per iteration there)."""
+    return 200 if p <= 200_000_000 else 100
 
 
 # Batched: 256 exponents log-uniform in [1e5, 5e6].

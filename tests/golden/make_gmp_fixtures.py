@@ -5,7 +5,16 @@ residue as ceil(p/32) little-endian uint32 words (s_0 = 4, unshifted).  The
 large exponents take GMP minutes to hours on one core, so the values are
 computed once here and committed as data.
 
-    python tests/golden/make_gmp_fixtures.py [--quick]
+    python tests/golden/make_gmp_fixtures.py [--quick] [--workers K]
+
+Jobs: the smooth-length sweep (iterations 200 and 2000), the large single
+exponent (every 100 iterations up to 1000), the two-sweep schedule cases,
+one length-coverage case per default FFT length above 1M words (the
+densest exponent the reference's policy assigns to it, workloads.densest_prime),
+and the learned selector's 512 evaluation primes after its 1000-iteration
+round-off test.
+Jobs run in K worker processes (default 1); the file is rewritten as each
+job ends, so an interrupted run keeps what it finished.
 """
 import hashlib
 import json
@@ -38,38 +47,74 @@ def entry(p, iters, words):
     }
 
 
+# default FFT lengths (words) of the reference's list above this get a
+# length-coverage case
+LENGTH_CASES_ABOVE = 1 << 20
+# iterations of the selector's round-off test (cudalucas_amd/selector.py CHECK_ITERS)
+SELECTOR_CHECK_ITERS = 1000
+
+
+def run_job(job):
+    p, marks, quick = job
+    state, done, rows = None, 0, []
+    t0 = time.time()
+    for m in marks:
+        state = O.gmp_ll(p, m - done, state)
+        done = m
+        rows.append(entry(p, m, state))
+        print("p=%d iter=%d res64=%s (%.0fs)" % (p, m, rows[-1]["res64"], time.time() - t0), flush=True)
+        if quick:
+            break
+    return p, rows
+
+
 def main():
     quick = "--quick" in sys.argv
+    workers = int(sys.argv[sys.argv.index("--workers") + 1]) if "--workers" in sys.argv else 1
     data = {}
     if os.path.exists(OUT):
         with open(OUT) as f:
             data = json.load(f)
     jobs = []
-    # smooth-length sweep exponents: prefix of 200 iterations
+    # smooth-length sweep exponents: prefixes of 200 and 2000 iterations
     for n in W.SMOOTH_LENGTHS:
-        jobs.append((W.smooth_prime(n), [200]))
+        jobs.append((W.smooth_prime(n), [200, W.SMOOTH_ITERS]))
     # large single exponent: checkpoints every 100 iterations up to 1000
     jobs.append((W.LARGE_P, list(range(100, W.LARGE_CHECK_ITER + 1, 100))))
     # two-sweep schedule cases of tests/test_gpu_parity.py above 5e6
     for p in SWEEP_FIXTURE_PRIMES:
         jobs.append((p, [SWEEP_ITERS]))
+    # one case per default length above 1M words, smallest first
+    for n in O.default_fft_lengths():
+        if n > LENGTH_CASES_ABOVE:
+            p = W.densest_prime(n)
+            jobs.append((p, [W.length_check_iters(p)]))
+    # the learned selector's evaluation primes: Res64 after the round-off
+    # test (1000 iterations from s_0 = 4)
+    for p in W.log_uniform_primes(W.SELECTOR_EVAL_SEED, W.SELECTOR_EVAL_COUNT, *W.SELECTOR_EVAL_RANGE):
+        jobs.append((p, [SELECTOR_CHECK_ITERS]))
+    todo = []
     for p, marks in jobs:
-        key = str(p)
-        have = {e["iterations"] for e in data.get(key, [])}
-        if all(m in have for m in marks):
-            continue
-        state, done, rows = None, 0, []
-        t0 = time.time()
-        for m in marks:
-            state = O.gmp_ll(p, m - done, state)
-            done = m
-            rows.append(entry(p, m, state))
-            print("p=%d iter=%d res64=%s (%.0fs)" % (p, m, rows[-1]["res64"], time.time() - t0), flush=True)
-            if quick:
-                break
-        data[key] = rows
+        have = {e["iterations"] for e in data.get(str(p), [])}
+        if not all(m in have for m in marks):
+            todo.append((p, marks, quick))
+
+    def save(p, rows):
+        old = {e["iterations"]: e for e in data.get(str(p), [])}
+        old.update({e["iterations"]: e for e in rows})
+        data[str(p)] = [old[k] for k in sorted(old)]
         with open(OUT, "w") as f:
             json.dump(data, f, indent=1, sort_keys=True)
+
+    if workers <= 1:
+        for job in todo:
+            save(*run_job(job))
+    else:
+        import multiprocessing as mp
+
+        with mp.Pool(workers) as pool:
+            for p, rows in pool.imap_unordered(run_job, todo):
+                save(p, rows)
 
 
 if __name__ == "__main__":

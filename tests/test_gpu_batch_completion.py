@@ -8,8 +8,9 @@ Mersenne-prime exponent ends at residue 0 and every other one does not;
 no iteration of any test exceeds the 0.35 roundoff limit (so there is no
 undetected failure to look for); and for the smallest exponents the full
 final residue equals the plain single-exponent engine's, run to completion
-beside it.  The run takes about half an hour of GPU time, so it is split in four
-parts and only runs when asked:
+beside it.  The whole run takes about half an hour of GPU time, so by default
+only the exponents below 1e6 run (with the Mersenne primes 110503 ...
+859433); the four parts of the whole run are asked for by name:
 
     LL_BATCH_COMPLETION=low  python -m pytest -s tests/test_gpu_batch_completion.py   # p < 2e6
     LL_BATCH_COMPLETION=mid  python -m pytest -s tests/test_gpu_batch_completion.py   # 2e6 <= p < 3.5e6
@@ -32,17 +33,17 @@ from conftest import announce
 pytestmark = pytest.mark.gpu
 
 MERSENNE_PRIME_EXPONENTS = [110503, 132049, 216091, 756839, 859433, 1257787, 1398269, 2976221, 3021377]
-PARTS = {"low": (0, 2_000_000), "mid": (2_000_000, 3_500_000), "high": (3_500_000, 4_300_000),
-         "top": (4_300_000, 10_000_000)}
+PARTS = {"default": (0, 1_000_000), "low": (0, 2_000_000), "mid": (2_000_000, 3_500_000),
+         "high": (3_500_000, 4_300_000), "top": (4_300_000, 10_000_000)}
 PLAIN_CHECKS = 24  # smallest exponents also run to completion in the plain engine
-PART = os.environ.get("LL_BATCH_COMPLETION", "")
+PART = os.environ.get("LL_BATCH_COMPLETION", "") or "default"
 
 
 def digest(words):
     return hashlib.sha256(np.ascontiguousarray(words, dtype="<u4").tobytes()).hexdigest()[:32]
 
 
-@pytest.mark.skipif(PART not in PARTS, reason="set LL_BATCH_COMPLETION=low|mid|high|top (4-10 GPU-minutes each)")
+@pytest.mark.skipif(PART not in PARTS, reason="LL_BATCH_COMPLETION must be one of %s" % "|".join(PARTS))
 def test_batch_to_completion():
     ps = W.log_uniform_primes(W.BATCH_SEED, W.BATCH_COUNT, *W.BATCH_RANGE)
     ps = sorted(set(ps) | set(MERSENNE_PRIME_EXPONENTS))
@@ -75,7 +76,7 @@ def test_batch_to_completion():
     print("part %s: %d exponents in %.1f s" % (PART, len(rows), wall), flush=True)
     # the plain engine on the smallest exponents, to completion
     plain = {}
-    for row in rows[:PLAIN_CHECKS] if PART == "low" else []:
+    for row in rows[:PLAIN_CHECKS] if PART in ("low", "default") else []:
         p, n = row["p"], row["n"]
         with L.LLContext(p, n) as c:
             c.load_initial(p % 1000)

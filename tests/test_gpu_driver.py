@@ -40,11 +40,23 @@ def gmp_res64(p, iters):
 
 
 def test_prime_exponent_reports_prime(tmp_path):
-    r = drive(["-f", "4096", "-x", "10000", "--offset", "99", 44497], tmp_path)
+    r = drive(["-f", "4096", "-x", "10000", "-c", "10000", "--offset", "99", 44497], tmp_path)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "M( 44497 )P" in r.stdout
     assert result_line(tmp_path).startswith("M( 44497 )P, offset = ")
-    assert not (tmp_path / "c44497").exists() or True
+    # a prime keeps its checkpoints (the reference removes them only for a
+    # nonzero residue, CUDALucas.cu:3175): the last one holds s_(k-1),
+    # shifted by its offset, for its next iteration k
+    p = 44497
+    words = np.frombuffer((tmp_path / ("c%d" % p)).read_bytes(), dtype=np.uint32)
+    end = O.nwords(p)
+    assert int(words[end]) == p and int(words[end + 1]) == 4096
+    assert int(words[end + 9]) == O.checkpoint_crc(words[:end + 9].tobytes())
+    nxt, off = int(words[end + 2]), int(words[end + 3])
+    assert 30000 < nxt < p - 2
+    value = O.words_to_int(words[:end])
+    unshifted = ((value >> off) | (value << (p - off))) & ((1 << p) - 1)
+    assert O.int_to_words(unshifted, p).tolist() == O.gmp_ll(p, nxt - 1).tolist()
 
 
 def test_composite_run_matches_gmp_and_reports(tmp_path):

@@ -166,7 +166,7 @@ def _selftest_cases():
     out = []
     for p, r in d["tests"] + d["testl"]:
         n, _ = L.choose_fft_length(p) if os.path.exists(L.LIB_PATH) else (0, 0)
-        if 0 < n <= 6291456:
+        if n > 0:
             out.append((p, int(r, 16), n))
     return out
 
@@ -210,17 +210,33 @@ def test_77m_at_4m_words_vs_gmp_fixture(gmp_fixtures, schedule):
 
 @pytest.mark.parametrize("n", W.SMOOTH_LENGTHS)
 def test_smooth_lengths_vs_gmp_fixture(n, gmp_fixtures):
+    """The smooth-length sweep: 2,000 iterations at ~18.5 bits/word, the
+    residue at 200 and 2,000 iterations equal to GMP's and, at 2,000, to the
+    power-of-two length's run of the same exponent."""
     p = W.smooth_prime(n)
-    fx = gmp_fixtures[(p, 200)]
-    announce("smooth length p=%d n=%d" % (p, n))
+    iters = W.SMOOTH_ITERS
+    announce("smooth length p=%d n=%d iters=%d" % (p, n, iters))
     with L.LLContext(p, n) as c:
         c.load_initial(0)
         m = c.iterate(200)
         w, r = c.residue()
+        fx = gmp_fixtures[(p, 200)]
+        assert "0x%016x" % r == fx["res64"] and sha(w) == fx["sha256"]
+        m = max(m, c.iterate(iters - 200))
+        w, r = c.residue()
+    fx = gmp_fixtures[(p, iters)]
     assert "0x%016x" % r == fx["res64"]
     assert sha(w) == fx["sha256"]
-    print("smooth length n=%d p=%d bits/word=%.3f max roundoff %.4f" % (n, p, p / n, m))
-    assert m < 0.45
+    n2 = W.SMOOTH_POW2[n]
+    announce("smooth length p=%d power-of-two n=%d iters=%d" % (p, n2, iters))
+    with L.LLContext(p, n2) as c:
+        c.load_initial(0)
+        m2 = c.iterate(iters)
+        w2, _ = c.residue()
+    assert (w2 == w).all()
+    print("smooth length n=%d p=%d bits/word=%.3f max roundoff %.4f (power of two %d: %.4f)"
+          % (n, p, p / n, m, n2, m2))
+    assert m < ROUNDOFF_LIMIT and m2 < ROUNDOFF_LIMIT
 
 
 SHORT_LINE_LENGTHS_K = [18, 20, 32, 36, 42, 48, 50, 56, 60, 64, 70, 80, 84, 96, 112, 120, 126, 144, 160,
@@ -241,3 +257,67 @@ def test_short_line_lengths_vs_gmp(nk):
         w, _ = c.residue()
     assert (w == O.gmp_ll(p, iters)).all(), info
     assert m < ROUNDOFF_LIMIT
+
+
+# ---------------------------------------------------- length coverage
+
+def _length_cases():
+    """Every default length this build supports above 1M words with its
+    committed GMP fixture: the densest exponent the reference's policy
+    assigns to the length."""
+    with open(os.path.join(GOLDEN, "gmp_residues.json")) as f:
+        fx = json.load(f)
+    out = []
+    for n in O.default_fft_lengths():
+        if n <= (1 << 20) or not (os.path.exists(L.LIB_PATH) and L.fft_supported(n)):
+            continue
+        p = W.densest_prime(n)
+        out.append(pytest.param(p, n, W.length_check_iters(p), id="%dK" % (n // 1024),
+                                marks=() if str(p) in fx else pytest.mark.skip(reason="no GMP fixture yet")))
+    return out
+
+
+@pytest.mark.parametrize("p,n,iters", _length_cases())
+def test_length_coverage_vs_gmp_fixture(p, n, iters, gmp_fixtures):
+    announce("length coverage p=%d n=%d iters=%d" % (p, n, iters))
+    with L.LLContext(p, n) as c:
+        info = c.info()
+        c.load_initial(0)
+        m = c.iterate(iters)
+        w, r = c.residue()
+    fx = gmp_fixtures[(p, iters)]
+    assert "0x%016x" % r == fx["res64"], info
+    assert sha(w) == fx["sha256"], info
+    print("length %dK p=%d bits/word=%.3f schedule W=%d H=%d R=%d max roundoff %.4f"
+          % (n // 1024, p, p / n, info["width"], info["height"], info["lines"], m))
+    assert m < ROUNDOFF_LIMIT
+
+
+# --------------------------------------------------- per-call overhead
+
+def test_single_iteration_call_is_cheap():
+    """ll_iterate(ctx, 1) at 4M words costs its kernels plus one
+    synchronisation: the per-iteration call the reference makes
+    (lucas_square, CUDALucas.cu:1869-1986) must not pay fixed host costs."""
+    import time
+
+    p, n = W.LARGE_P, W.LARGE_N
+    announce("single-iteration call cost p=%d n=%d" % (p, n))
+    with L.LLContext(p, n) as c:
+        c.load_initial(0)
+        c.iterate(64)
+        walls, gpus = [], []
+        for _ in range(60):
+            t0 = time.perf_counter()
+            c.iterate(1)
+            walls.append(time.perf_counter() - t0)
+            gpus.append(c.last_iterate_ms() * 1e-3)
+        t0 = time.perf_counter()
+        c.iterate(20)
+        wall20 = time.perf_counter() - t0
+        gpu20 = c.last_iterate_ms() * 1e-3
+    walls, gpus = np.array(walls[10:]), np.array(gpus[10:])
+    print("ll_iterate(1): median wall %.1f us, GPU %.1f us; ll_iterate(20): wall %.1f us/iter, GPU %.1f us/iter"
+          % (np.median(walls) * 1e6, np.median(gpus) * 1e6, wall20 / 20 * 1e6, gpu20 / 20 * 1e6))
+    assert np.median(walls) < 1e-4
+    assert wall20 < 1.15 * gpu20 + 1e-4

@@ -152,48 +152,20 @@ def test_selector_without_times_never_exceeds_reference_length():
 
 
 @needs_model
-def test_selector_saves_time_on_seeded_exponents_with_exact_residues():
-    """A 64-exponent slice of the 512 seeded evaluation primes in [1e5, 2e7]:
-    the selected lengths save projected one-at-a-time time and measured
-    batched time against the reference policy, every residue after the
-    round-off test equals the reference-length run's, and Res64 equals GMP
-    for the exponents below 2e6."""
-    primes = W.log_uniform_primes(W.SELECTOR_EVAL_SEED, W.SELECTOR_EVAL_COUNT, *W.SELECTOR_EVAL_RANGE)[::8]
-    times = S.read_times(S.TIMES_PATH)
-    announce("selector evaluation on %d exponents" % len(primes))
-    from cudalucas_amd import tuner as T
-
-    btimes = S.read_times(S.BATCH_TIMES_PATH) if os.path.exists(S.BATCH_TIMES_PATH) else None
-    sched = T.load(T.BATCH_TABLE) if os.path.exists(T.BATCH_TABLE) else None
-    rep = S.evaluate(S.MODEL_PATH, primes, times, batch_times=btimes, schedules=sched, batch_iters=256)
-    print({k: v for k, v in rep.items() if k != "per_exponent"})
-    for form in ("timed", "batched"):
-        assert rep[form]["residue_mismatches_vs_reference_length"] == []
-        assert rep[form]["max_roundoff"] <= S.LIMIT
-    assert rep["reference_length_check_failures"] == 0
-    assert rep["time_saved_fraction_plain"] > 0.0
-    # the batched engine at the selected lengths against the plain
-    # per-exponent path at the reference policy's lengths
-    assert rep["batched_selected_vs_plain_reference"] > 2.0
-    for row in rep["per_exponent"]:
-        if row["p"] < 2_000_000:
-            g = O.gmp_ll(row["p"], S.CHECK_ITERS)
-            assert "%016x" % O.res64(g) == row["res64"], row
-
-
-@needs_model
-@pytest.mark.skipif(os.environ.get("LL_FULL_SELECTOR_EVAL") != "1",
-                    reason="full 512-exponent evaluation: set LL_FULL_SELECTOR_EVAL=1 (about 5 minutes)")
-def test_selector_full_evaluation_all_residues_vs_gmp(tmp_path):
-    """All 512 seeded evaluation primes in [1e5, 2e7]: zero fallbacks left
-    unresolved, zero residue mismatches, and Res64 after the round-off test
-    equal to GMP for every exponent (GMP on a process pool)."""
+def test_selector_full_evaluation_all_residues_vs_gmp(gmp_fixtures):
+    """All 512 seeded evaluation primes in [1e5, 2e7]: the selected lengths
+    save projected one-at-a-time time and measured batched time against the
+    reference policy, every residue after the round-off test equals the
+    reference-length run's, there is no unresolved fallback, and every
+    Res64 after the round-off test equals GMP's (committed fixtures,
+    tests/golden/make_gmp_fixtures.py)."""
     import json
-    from multiprocessing import Pool
 
     from cudalucas_amd import tuner as T
 
     primes = W.log_uniform_primes(W.SELECTOR_EVAL_SEED, W.SELECTOR_EVAL_COUNT, *W.SELECTOR_EVAL_RANGE)
+    missing = [p for p in primes if (p, S.CHECK_ITERS) not in gmp_fixtures]
+    assert not missing, "missing GMP fixtures for %d exponents" % len(missing)
     times = S.read_times(S.TIMES_PATH)
     btimes = S.read_times(S.BATCH_TIMES_PATH)
     sched = T.load(T.BATCH_TABLE)
@@ -203,15 +175,16 @@ def test_selector_full_evaluation_all_residues_vs_gmp(tmp_path):
     if out:
         with open(out, "w") as f:
             json.dump(rep, f, indent=1)
+    print({k: v for k, v in rep.items() if k != "per_exponent"})
     for form in ("timed", "batched"):
         assert rep[form]["residue_mismatches_vs_reference_length"] == []
-    with Pool(16) as pool:
-        want = pool.starmap(_gmp_res64, [(p, S.CHECK_ITERS) for p in primes])
+        assert rep[form]["max_roundoff"] <= S.LIMIT
+    assert rep["reference_length_check_failures"] == 0
+    assert rep["time_saved_fraction_plain"] > 0.0
+    # the batched engine at the selected lengths against the plain
+    # per-exponent path at the reference policy's lengths
+    assert rep["batched_selected_vs_plain_reference"] > 2.0
     got = {row["p"]: row["res64"] for row in rep["per_exponent"]}
-    bad = [p for p, w in zip(primes, want) if got[p] != w]
+    bad = [p for p in primes if "0x" + got[p] != gmp_fixtures[(p, S.CHECK_ITERS)]["res64"]]
     print("GMP Res64 checked for %d exponents, %d mismatches" % (len(primes), len(bad)))
     assert bad == []
-
-
-def _gmp_res64(p, iters):
-    return "%016x" % O.res64(O.gmp_ll(p, iters))
