@@ -88,6 +88,7 @@ def lib():
     L.ll_destroy.restype = None
     L.ll_load_digits.argtypes = [P, P, u32]
     L.ll_iterate.argtypes = [P, u32, P, ctypes.POINTER(f32)]
+    L.ll_iterate_noalloc.argtypes = [P, u32, P, ctypes.POINTER(f32)]
     L.ll_read_digits.argtypes = [P, P, ctypes.POINTER(u32)]
     L.ll_reset_error.argtypes = [P]
     L.ll_error_max.argtypes = [P]
@@ -384,6 +385,15 @@ class LLContext:
         errs = np.zeros(count, dtype=np.float32) if per_iter else None
         m = ctypes.c_float()
         self._check(lib().ll_iterate(self._h, count, None if errs is None else _ptr(errs), ctypes.byref(m)))
+        return (m.value, errs) if per_iter else m.value
+
+    def iterate_noalloc(self, count, per_iter=False):
+        """ll_iterate_noalloc: the same iterations with the shuttle and error
+        slots in the device's static scratch array."""
+        errs = np.zeros(count, dtype=np.float32) if per_iter else None
+        m = ctypes.c_float()
+        self._check(lib().ll_iterate_noalloc(self._h, count, None if errs is None else _ptr(errs),
+                                             ctypes.byref(m)))
         return (m.value, errs) if per_iter else m.value
 
     def read_digits(self):

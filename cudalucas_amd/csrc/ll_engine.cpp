@@ -18,6 +18,8 @@ using namespace llamd;
 
 namespace {
 
+static_assert(LL_NOALLOC_SCRATCH_BYTES == kNoAllocScratchBytes, "the header states the scratch size");
+
 constexpr uint32_t kRingSize = 1u << 16;   // per-iteration error slots
 constexpr uint32_t kGraphIters = 32;       // iterations per captured graph
 constexpr uint32_t kTwoLevelShift = 11;    // 2048-entry fine root table
@@ -140,6 +142,11 @@ struct ll_ctx {
   PlanArgs args{};
   PlanArgs *d_args = nullptr;         // device copy of args
   PlanArgs *d_args_unload = nullptr;  // args with err -> unload_err
+  // the no-allocation path: args whose shuttle and error slots are in the
+  // device's static scratch array (null if the shuttle does not fit)
+  PlanArgs *d_args_static = nullptr;
+  uint8_t *scratch = nullptr;
+  size_t shuttle_bytes = 0;
   uint32_t ring_size = kRingSize;
   uint32_t *ring = nullptr;
   DevState *st = nullptr;
@@ -243,6 +250,7 @@ static int build_sweeps(ll_ctx *c) {
   if ((rc = upload(c, (double **)&A.iwB, iwB))) return rc;
   if ((rc = upload(c, (int32_t **)&A.fB, fB))) return rc;
   const size_t shuttle = (size_t)(H / R) * W;
+  c->shuttle_bytes = shuttle * sizeof(uint64_t);
   if ((rc = dalloc(c, &A.shuttle, shuttle))) return rc;
   HIPCHK(c, hipMemset(A.shuttle, 0xff, shuttle * sizeof(uint64_t)));
   if ((rc = dalloc(c, &c->st, 1))) return rc;
@@ -269,6 +277,14 @@ static int build_sweeps(ll_ctx *c) {
   PlanArgs U = A;
   U.err = c->unload_err;
   HIPCHK(c, hipMemcpy(c->d_args_unload, &U, sizeof(PlanArgs), hipMemcpyHostToDevice));
+  if (c->shuttle_bytes <= kNoAllocShuttleBytes) {
+    PlanArgs S = A;
+    S.shuttle = (uint64_t *)c->scratch;
+    S.err = (uint32_t *)(c->scratch + kNoAllocShuttleBytes);
+    S.err_mask = kNoAllocRing - 1;
+    if ((rc = dalloc(c, &c->d_args_static, 1))) return rc;
+    HIPCHK(c, hipMemcpy(c->d_args_static, &S, sizeof(PlanArgs), hipMemcpyHostToDevice));
+  }
   return LL_OK;
 }
 
@@ -380,6 +396,12 @@ static int ctx_create(uint32_t p, uint32_t n, const ll_options *opts, uint32_t r
   if (!rc && (hipHostMalloc((void **)&c->h_ring, sizeof(uint32_t) * ring_size, 0) != hipSuccess ||
               hipHostMalloc((void **)&c->h_words, sizeof(HostWords), 0) != hipSuccess))
     rc = fail(c, LL_ERR_ALLOC, "hipHostMalloc of the readback buffers failed");
+  if (!rc) {
+    void *sp = nullptr;
+    e = noalloc_scratch(c->device, &sp);
+    if (e != hipSuccess) rc = fail(c, LL_ERR_HIP, "no-allocation scratch lookup: %s", hipGetErrorString(e));
+    c->scratch = (uint8_t *)sp;
+  }
   if (!rc) rc = dalloc(c, &c->x, c->m);
   if (!rc) rc = dalloc(c, &c->digits, n);
   if (!rc) rc = dalloc(c, &c->unload_err, 1);
@@ -425,32 +447,37 @@ int ll_load_digits(ll_ctx *c, const int32_t *digits, uint32_t offset) {
   return LL_OK;
 }
 
-// Both iterate forms run `count` <= ring_size iterations, then bring back
-// only the error slots those iterations wrote, and the offset, through
-// pinned host memory: a call costs its kernels plus one synchronisation.
-static int iterate_resident(ll_ctx *c, uint32_t count, float *errs) {
+// Both iterate forms run `count` iterations (no more than the error slots
+// they are given), then bring back only the error slots those iterations
+// wrote, and the offset, through pinned host memory (c->h_ring, c->h_words):
+// a call costs its kernels plus one synchronisation.
+static int iterate_resident(ll_ctx *c, uint32_t count, float *dev_err) {
   HostWords &hw = *c->h_words;
   hw.rdesc = c->rdesc;
   hw.rdesc.iters = count;
+  hw.rdesc.err = dev_err;
   HIPCHK(c, hipMemcpyAsync(c->d_rdesc, &hw.rdesc, sizeof(ResidentDesc), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
   HIPCHK(c, launch_resident(c->sch.rk, c->d_rdesc, 1, c->stream));
   HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-  HIPCHK(c, hipMemcpyAsync(c->h_ring, c->r_err, sizeof(float) * count, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->h_ring, dev_err, sizeof(float) * count, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(&hw.offset, c->r_offset, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   float ms = 0.f;
   HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
   c->last_gpu_ms += ms;
-  memcpy(errs, c->h_ring, sizeof(float) * count);
   c->offset = hw.offset;
   return LL_OK;
 }
 
-static int iterate_sweeps(ll_ctx *c, uint32_t count, float *errs) {
+// dargs: the PlanArgs the kernels read; ring/ring_size: the error slots
+// those args name; graph: replay the context's captured graph (built on
+// c->d_args) for whole groups of kGraphIters iterations.
+static int iterate_sweeps(ll_ctx *c, uint32_t count, const PlanArgs *dargs, const uint32_t *ring,
+                          uint32_t ring_size, bool graph) {
   HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
   uint32_t left = count;
-  if (c->use_graph && count >= kGraphIters) {
+  if (graph && c->use_graph && count >= kGraphIters) {
     int rc = build_graph(c);
     if (rc) return rc;
     while (left >= kGraphIters) {
@@ -459,16 +486,16 @@ static int iterate_sweeps(ll_ctx *c, uint32_t count, float *errs) {
     }
   }
   for (; left; left--) {
-    HIPCHK(c, launch_square(c->sch.sk, c->d_args, 1, c->sch.W, c->stream));
-    HIPCHK(c, launch_carry(c->sch.ck, c->d_args, 1, c->sch.H, c->stream));
+    HIPCHK(c, launch_square(c->sch.sk, dargs, 1, c->sch.W, c->stream));
+    HIPCHK(c, launch_carry(c->sch.ck, dargs, 1, c->sch.H, c->stream));
   }
   HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
   // the slots of epochs epoch .. epoch + count - 1, which may wrap the ring
-  const uint32_t first = (uint32_t)(c->epoch & (c->ring_size - 1));
-  const uint32_t n1 = std::min(count, c->ring_size - first);
-  HIPCHK(c, hipMemcpyAsync(c->h_ring, c->ring + first, sizeof(uint32_t) * n1, hipMemcpyDeviceToHost, c->stream));
+  const uint32_t first = (uint32_t)(c->epoch & (ring_size - 1));
+  const uint32_t n1 = std::min(count, ring_size - first);
+  HIPCHK(c, hipMemcpyAsync(c->h_ring, ring + first, sizeof(uint32_t) * n1, hipMemcpyDeviceToHost, c->stream));
   if (n1 < count)
-    HIPCHK(c, hipMemcpyAsync(c->h_ring + n1, c->ring, sizeof(uint32_t) * (count - n1), hipMemcpyDeviceToHost,
+    HIPCHK(c, hipMemcpyAsync(c->h_ring + n1, ring, sizeof(uint32_t) * (count - n1), hipMemcpyDeviceToHost,
                              c->stream));
   HIPCHK(c, hipMemcpyAsync(&c->h_words->offset, &c->st->offset, sizeof(uint32_t), hipMemcpyDeviceToHost,
                            c->stream));
@@ -476,13 +503,16 @@ static int iterate_sweeps(ll_ctx *c, uint32_t count, float *errs) {
   float ms = 0.f;
   HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
   c->last_gpu_ms += ms;
-  memcpy(errs, c->h_ring, sizeof(float) * count);
   c->epoch += count;
   c->offset = c->h_words->offset;
   return LL_OK;
 }
 
-int ll_iterate(ll_ctx *c, uint32_t count, float *per_iter, float *max_err) {
+// ll_iterate and ll_iterate_noalloc: the same kernels, with the carry
+// shuttle and error slots either the context's own or the device's static
+// scratch array.  Neither allocates; the no-allocation form also replays no
+// graph (direct launches only).
+static int iterate_common(ll_ctx *c, uint32_t count, float *per_iter, float *max_err, bool noalloc) {
   if (!c) return LL_ERR_ARG;
   if (!c->loaded) return fail(c, LL_ERR_STATE, "ll_iterate before ll_load_digits");
   HIPCHK(c, hipSetDevice(c->device));
@@ -490,14 +520,34 @@ int ll_iterate(ll_ctx *c, uint32_t count, float *per_iter, float *max_err) {
     c->inject_kind = 0;
     return fail(c, LL_ERR_HIP, "injected device error");
   }
-  std::vector<float> errs(std::min<uint32_t>(count, c->ring_size));
+  uint32_t slots = c->ring_size;
+  if (noalloc) {
+    slots = std::min(slots, kNoAllocRing);
+    if (c->sch.path == 1) {
+      if (!c->d_args_static)
+        return fail(c, LL_ERR_ARG, "carry shuttle of %zu bytes exceeds the %zu-byte no-allocation scratch",
+                    c->shuttle_bytes, kNoAllocShuttleBytes);
+      // stale carries of other contexts must not carry this epoch's tag
+      HIPCHK(c, launch_fill((uint32_t *)c->scratch, 0xffffffffu, c->shuttle_bytes / 4, c->stream));
+    }
+  }
   c->last_gpu_ms = 0.0;
   float m = 0.f;
   uint32_t done = 0;
   while (done < count) {
-    const uint32_t chunk = std::min<uint32_t>(count - done, c->ring_size);
-    int rc = c->sch.path == 0 ? iterate_resident(c, chunk, errs.data()) : iterate_sweeps(c, chunk, errs.data());
+    const uint32_t chunk = std::min<uint32_t>(count - done, slots);
+    int rc;
+    if (c->sch.path == 0) {
+      float *dev_err = noalloc ? (float *)(c->scratch + kNoAllocShuttleBytes) : c->r_err;
+      rc = iterate_resident(c, chunk, dev_err);
+    } else if (noalloc) {
+      rc = iterate_sweeps(c, chunk, c->d_args_static, (const uint32_t *)(c->scratch + kNoAllocShuttleBytes),
+                          kNoAllocRing, false);
+    } else {
+      rc = iterate_sweeps(c, chunk, c->d_args, c->ring, c->ring_size, true);
+    }
     if (rc) return rc;
+    float *errs = (float *)c->h_ring;
     if (c->inject_kind == 1) {
       if (c->inject_after < chunk) {
         errs[c->inject_after] = c->inject_value;
@@ -507,13 +557,21 @@ int ll_iterate(ll_ctx *c, uint32_t count, float *per_iter, float *max_err) {
       }
     }
     for (uint32_t i = 0; i < chunk; i++) m = std::max(m, errs[i]);
-    if (per_iter) memcpy(per_iter + done, errs.data(), sizeof(float) * chunk);
+    if (per_iter) memcpy(per_iter + done, errs, sizeof(float) * chunk);
     done += chunk;
     c->iterations += chunk;
   }
   c->max_err = std::max(c->max_err, m);
   if (max_err) *max_err = m;
   return LL_OK;
+}
+
+int ll_iterate(ll_ctx *c, uint32_t count, float *per_iter, float *max_err) {
+  return iterate_common(c, count, per_iter, max_err, false);
+}
+
+int ll_iterate_noalloc(ll_ctx *c, uint32_t count, float *per_iter, float *max_err) {
+  return iterate_common(c, count, per_iter, max_err, true);
 }
 
 int ll_read_digits(ll_ctx *c, int32_t *digits, uint32_t *offset) {
@@ -552,11 +610,10 @@ int ll_profile_kernels(ll_ctx *c, uint32_t count, float *kernel_ms) {
   HIPCHK(c, hipSetDevice(c->device));
   kernel_ms[0] = kernel_ms[1] = 0.f;
   if (c->sch.path == 0) {
-    std::vector<float> errs(std::min<uint32_t>(count, c->ring_size));
     for (uint32_t done = 0; done < count;) {
       const uint32_t chunk = std::min<uint32_t>(count - done, c->ring_size);
       c->last_gpu_ms = 0.0;
-      int rc = iterate_resident(c, chunk, errs.data());
+      int rc = iterate_resident(c, chunk, c->r_err);
       if (rc) return rc;
       kernel_ms[0] += (float)c->last_gpu_ms;
       done += chunk;

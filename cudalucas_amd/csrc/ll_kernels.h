@@ -83,6 +83,17 @@ struct ResidentKernel {
   uint32_t threads;
 };
 
+// The no-allocation iteration path (ll_iterate_noalloc) keeps the carry
+// shuttle and the per-iteration error slots in one file-scope device array
+// of this many bytes per GPU: the last kNoAllocRing * 4 bytes are the error
+// slots, the rest the shuttle.
+constexpr size_t kNoAllocScratchBytes = (size_t)16 << 20;
+constexpr uint32_t kNoAllocRing = 16384;
+constexpr size_t kNoAllocShuttleBytes = kNoAllocScratchBytes - (size_t)kNoAllocRing * 4;
+// Device address of that array on `device`, which must be the current
+// device; looked up once per device (hipGetSymbolAddress) and kept.
+hipError_t noalloc_scratch(int device, void **ptr);
+
 const SquareKernel *find_square(uint32_t H);
 const CarryKernel *find_carry(uint32_t W, uint32_t R);
 const CarryKernel *carry_table(size_t *count);

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
 #include <vector>
 
 #include "ll_sweeps.h"
@@ -156,6 +157,33 @@ __global__ void fill_u32(uint32_t *p, uint32_t v, size_t n) {
 template <int H>
 __global__ __launch_bounds__(256) void column_pass_table(const double *qc, cplx *out) {
   build_wave_table<H, 256, false>(out, QuarterRoots<H>{qc});
+}
+
+// ------------------------------------------------------- static scratch
+// The no-allocation iteration path's shuttle and error slots.  In host code
+// the name `scratch` is only a stand-in: kernels are given the address that
+// hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+hipError_t noalloc_scratch(int device, void **ptr) {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static void *addr[kMaxDevices] = {nullptr};
+  if (!ptr || device < 0 || device >= kMaxDevices) return hipErrorInvalidValue;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[device]) {
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return e;
+    if (cur != device) return hipErrorInvalidDevice;
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    addr[device] = p;
+  }
+  *ptr = addr[device];
+  return hipSuccess;
 }
 
 // ------------------------------------------------------------- dispatch

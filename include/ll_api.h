@@ -84,6 +84,20 @@ int ll_load_digits(ll_ctx *ctx, const int32_t *digits, uint32_t offset);
  * 32-bit carry shuttle: the iteration must be treated as failed. */
 int ll_iterate(ll_ctx *ctx, uint32_t count, float *per_iter_err, float *max_err);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory while they run (serving loops, harnesses that forbid hipMalloc):
+ * ll_iterate with the carry shuttle and the per-iteration error slots in a
+ * file-scope device array of LL_NOALLOC_SCRATCH_BYTES per GPU instead of
+ * the context's own buffers.  It calls no hipMalloc/hipFree and replays no
+ * graph (direct kernel launches; results identical to ll_iterate).  Limit:
+ * the schedule's shuttle, (H / R) * W * 8 bytes, must fit in
+ * LL_NOALLOC_SCRATCH_BYTES - 64 KiB (LL_ERR_ARG otherwise; e.g. 2 MiB at
+ * 4M words).  Every call on a GPU shares that GPU's array, which keeps the
+ * last call's contents: each call clears the shuttle first, and calls that
+ * overlap on one GPU (another thread, another context) race. */
+#define LL_NOALLOC_SCRATCH_BYTES (16u << 20)
+int ll_iterate_noalloc(ll_ctx *ctx, uint32_t count, float *per_iter_err, float *max_err);
+
 /* Replaces cudaMemcpy(x_int, g_xint) (CUDALucas.cu:2773, 2910, 2963, 3112,
  * 3155): n int32 digits of the current shifted residue and its offset. */
 int ll_read_digits(ll_ctx *ctx, int32_t *digits, uint32_t *offset);
