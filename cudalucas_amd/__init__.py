@@ -26,6 +26,7 @@ LL_ERR_ROUNDOFF = -6
 
 LL_OPT_FORCE_SWEEPS = 1
 LL_OPT_NO_GRAPH = 2
+LL_OPT_GENERIC = 4
 
 _lib = None
 

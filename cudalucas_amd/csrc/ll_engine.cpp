@@ -30,7 +30,74 @@ struct Schedule {
   const ResidentKernel *rk;
   const SquareKernel *sk;
   const CarryKernel *ck;
+  bool generic;   // the runtime-plan kernels (ll_generic.hip)
 };
+
+bool seven_smooth(uint32_t x) {
+  if (!x) return false;
+  for (uint32_t f : {2u, 3u, 5u, 7u})
+    while (x % f == 0) x /= f;
+  return x == 1;
+}
+
+// Radix passes of the generic plan of L: the odd prime factors, then the
+// power of two in radix-16 passes with an 8, 4 or 2 remainder first.
+// Empty if L is not 7-smooth or needs more than kGenMaxPasses passes.
+std::vector<uint32_t> generic_radices(uint32_t L) {
+  std::vector<uint32_t> r;
+  uint32_t x = L;
+  for (uint32_t f : {7u, 5u, 3u})
+    while (x % f == 0) {
+      r.push_back(f);
+      x /= f;
+    }
+  int k = 0;
+  while (x % 2 == 0) {
+    x /= 2;
+    k++;
+  }
+  if (x != 1) return {};
+  if (k % 4) r.push_back(1u << (k % 4));
+  for (int i = 0; i < k / 4; i++) r.push_back(16);
+  if (r.size() > (size_t)kGenMaxPasses) return {};
+  return r;
+}
+
+// The generic schedule of M complex points, if any: H | M columns of 64 to
+// 4800 points (two fit in LDS), W = M / H even lines, R a power of two
+// lines per carry block whose LDS fits; o may force W, H or R.  Preferred:
+// H near 2048, R = 8 (128-byte column runs) or 4, fewer line passes.
+bool pick_generic(uint32_t M, uint32_t want_r, const ll_options *o, Schedule *s) {
+  double best = 1e30;
+  bool found = false;
+  for (uint32_t H = 64; H <= 4800; H += 2) {
+    if (M % H || !seven_smooth(H) || square_generic_lds(H) > kGenMaxLds) continue;
+    const uint32_t W = M / H;
+    if (W % 2 || W < 8 || !seven_smooth(W)) continue;
+    if (o && o->width && o->width != W) continue;
+    if (o && o->height && o->height != H) continue;
+    if (generic_radices(W).empty() || generic_radices(H).empty()) continue;
+    for (uint32_t R = 16; R >= 1; R /= 2) {
+      if (H % R || carry_generic_lds(W, R) > kGenMaxLds) continue;
+      if (want_r && want_r != R) continue;
+      double cost = fabs(log2((double)H) - 11.0) * 2.0;
+      cost += R == 8 ? 0.0 : R == 4 ? 0.5 : R == 16 ? 0.7 : R == 2 ? 1.5 : 4.0;
+      cost += 0.1 * (double)generic_radices(W).size();
+      if (cost < best) {
+        best = cost;
+        found = true;
+        s->path = 1;
+        s->W = W;
+        s->H = H;
+        s->R = R;
+        s->generic = true;
+        s->sk = generic_square();
+        s->ck = generic_carry();
+      }
+    }
+  }
+  return found;
+}
 
 // Preferred schedules installed by ll_schedule_table_set (a tuning file).
 std::vector<ll_schedule> &schedule_table() {
@@ -42,7 +109,7 @@ bool pick_schedule_auto(uint32_t n, const ll_options *o, Schedule *s);
 
 bool pick_schedule(uint32_t n, const ll_options *o, Schedule *s) {
   const bool explicit_choice = o && (o->width || o->height || o->lines || o->threads[1] > 0 ||
-                                     (o->flags & LL_OPT_FORCE_SWEEPS));
+                                     (o->flags & (LL_OPT_FORCE_SWEEPS | LL_OPT_GENERIC)));
   if (!explicit_choice) {
     for (const ll_schedule &e : schedule_table()) {
       if (e.n != n) continue;
@@ -67,7 +134,7 @@ bool pick_schedule_auto(uint32_t n, const ll_options *o, Schedule *s) {
   if (n < 512 || n % 2) return false;
   const uint32_t M = n / 2;
   const uint32_t flags = o ? o->flags : 0;
-  if (!(flags & LL_OPT_FORCE_SWEEPS) && !(o && (o->width || o->height))) {
+  if (!(flags & (LL_OPT_FORCE_SWEEPS | LL_OPT_GENERIC)) && !(o && (o->width || o->height))) {
     if ((s->rk = find_resident(M))) {
       s->path = 0;
       return true;
@@ -79,6 +146,7 @@ bool pick_schedule_auto(uint32_t n, const ll_options *o, Schedule *s) {
     want_r = (uint32_t)o->threads[1] / 32;
     want_r = want_r <= 2 ? 2 : want_r >= 8 ? 8 : 4;
   }
+  if (flags & LL_OPT_GENERIC) return pick_generic(M, want_r, o, s);
   size_t count = 0;
   const CarryKernel *tab = carry_table(&count);
   double best = 1e30;
@@ -105,7 +173,8 @@ bool pick_schedule_auto(uint32_t n, const ll_options *o, Schedule *s) {
       s->sk = sk;
     }
   }
-  return s->ck != nullptr;
+  if (s->ck) return true;
+  return pick_generic(M, want_r, o, s);
 }
 
 }  // namespace
@@ -200,18 +269,54 @@ static int upload(ll_ctx *c, T **ptr, const std::vector<T> &v) {
   return LL_OK;
 }
 
+// Tables of one generic plan: radices, roots exp(-2 pi i e/L), and the
+// digit reversal of its in-place forward transform (position q = sum k_p
+// s_p, s_p = L / (r_0 ... r_p), holds frequency sum k_p r_0 ... r_(p-1)).
+static int build_generic_line(ll_ctx *c, uint32_t L, GenLine *g) {
+  const std::vector<uint32_t> rad = generic_radices(L);
+  if (rad.empty()) return fail(c, LL_ERR_UNSUPPORTED, "no generic plan for length %u", L);
+  memset(g, 0, sizeof(*g));
+  g->L = L;
+  g->np = (uint32_t)rad.size();
+  for (size_t i = 0; i < rad.size(); i++) g->radix[i] = (uint8_t)rad[i];
+  std::vector<double> roots(2 * (size_t)L);
+  for (uint32_t e = 0; e < L; e++) root_of_unity(e, L, &roots[2 * e], &roots[2 * e + 1]);
+  std::vector<uint32_t> perm(L), iperm(L);
+  for (uint32_t q = 0; q < L; q++) {
+    uint32_t span = L, place = 1, f = 0;
+    for (uint32_t r : rad) {
+      span /= r;
+      f += ((q / span) % r) * place;
+      place *= r;
+    }
+    perm[q] = f;
+    iperm[f] = q;
+  }
+  int rc;
+  if ((rc = upload(c, (double **)&g->roots, roots))) return rc;
+  if ((rc = upload(c, (uint32_t **)&g->perm, perm))) return rc;
+  if ((rc = upload(c, (uint32_t **)&g->iperm, iperm))) return rc;
+  return LL_OK;
+}
+
 static int build_sweeps(ll_ctx *c) {
   const uint32_t p = c->p, n = c->n, M = c->m, W = c->sch.W, H = c->sch.H, R = c->sch.R;
   PlanArgs &A = c->args;
   std::vector<double> qc;
   int rc;
+  memset(&A.gW, 0, sizeof(A.gW));
+  memset(&A.gH, 0, sizeof(A.gH));
+  if (c->sch.generic) {
+    if ((rc = build_generic_line(c, W, &A.gW))) return rc;
+    if ((rc = build_generic_line(c, H, &A.gH))) return rc;
+  }
   quarter_cosines(W, qc);
   if ((rc = upload(c, (double **)&A.qcW, qc))) return rc;
   quarter_cosines(H, qc);
   if ((rc = upload(c, (double **)&A.qcH, qc))) return rc;
   cplx *ptH = nullptr;
   if ((rc = dalloc(c, &ptH, H))) return rc;
-  HIPCHK(c, launch_square_table(c->sch.sk, A.qcH, ptH, c->stream));
+  if (c->sch.sk->table) HIPCHK(c, launch_square_table(c->sch.sk, A.qcH, ptH, c->stream));
   A.ptH = ptH;
   A.ptW = nullptr;
   if (c->sch.ck->table) {
@@ -332,8 +437,8 @@ static int build_graph(ll_ctx *c) {
   hipGraph_t g = nullptr;
   HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   for (uint32_t i = 0; i < kGraphIters; i++) {
-    (void)launch_square(c->sch.sk, c->d_args, 1, c->sch.W, c->stream);
-    (void)launch_carry(c->sch.ck, c->d_args, 1, c->sch.H, c->stream);
+    (void)launch_square(c->sch.sk, c->d_args, 1, c->sch.W, c->sch.H, c->stream);
+    (void)launch_carry(c->sch.ck, c->d_args, 1, c->sch.W, c->sch.H, c->sch.R, c->stream);
   }
   HIPCHK(c, hipStreamEndCapture(c->stream, &g));
   hipError_t e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
@@ -436,7 +541,7 @@ int ll_load_digits(ll_ctx *c, const int32_t *digits, uint32_t offset) {
                              c->stream));
     HIPCHK(c, launch_resident_load(c->d_rdesc, 1, c->digits, c->stream));
   } else {
-    HIPCHK(c, launch_load(c->sch.ck, c->d_args, 1, c->sch.H, c->stream));
+    HIPCHK(c, launch_load(c->sch.ck, c->d_args, 1, c->sch.W, c->sch.H, c->sch.R, c->stream));
     HIPCHK(c, hipMemcpyAsync(&c->st->offset, &c->h_words->offset, sizeof(uint32_t), hipMemcpyHostToDevice,
                              c->stream));
   }
@@ -486,8 +591,8 @@ static int iterate_sweeps(ll_ctx *c, uint32_t count, const PlanArgs *dargs, cons
     }
   }
   for (; left; left--) {
-    HIPCHK(c, launch_square(c->sch.sk, dargs, 1, c->sch.W, c->stream));
-    HIPCHK(c, launch_carry(c->sch.ck, dargs, 1, c->sch.H, c->stream));
+    HIPCHK(c, launch_square(c->sch.sk, dargs, 1, c->sch.W, c->sch.H, c->stream));
+    HIPCHK(c, launch_carry(c->sch.ck, dargs, 1, c->sch.W, c->sch.H, c->sch.R, c->stream));
   }
   HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
   // the slots of epochs epoch .. epoch + count - 1, which may wrap the ring
@@ -587,7 +692,7 @@ int ll_read_digits(ll_ctx *c, int32_t *digits, uint32_t *offset) {
     HIPCHK(c, launch_resident_unload(c->d_rdesc, 1, c->digits, c->unload_err, c->stream));
     HIPCHK(c, hipMemcpyAsync(&hw.offset, c->r_offset, 4, hipMemcpyDeviceToHost, c->stream));
   } else {
-    HIPCHK(c, launch_unload(c->sch.ck, c->d_args_unload, 1, c->sch.H, c->stream));
+    HIPCHK(c, launch_unload(c->sch.ck, c->d_args_unload, 1, c->sch.W, c->sch.H, c->sch.R, c->stream));
     HIPCHK(c, hipMemcpyAsync(&hw.offset, &c->st->offset, 4, hipMemcpyDeviceToHost, c->stream));
   }
   HIPCHK(c, hipMemcpyAsync(digits, c->digits, sizeof(int32_t) * c->n, hipMemcpyDeviceToHost, c->stream));
@@ -625,9 +730,9 @@ int ll_profile_kernels(ll_ctx *c, uint32_t count, float *kernel_ms) {
   double sq = 0.0, ca = 0.0;
   for (uint32_t i = 0; i < count; i++) {
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-    HIPCHK(c, launch_square(c->sch.sk, c->d_args, 1, c->sch.W, c->stream));
+    HIPCHK(c, launch_square(c->sch.sk, c->d_args, 1, c->sch.W, c->sch.H, c->stream));
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-    HIPCHK(c, launch_carry(c->sch.ck, c->d_args, 1, c->sch.H, c->stream));
+    HIPCHK(c, launch_carry(c->sch.ck, c->d_args, 1, c->sch.W, c->sch.H, c->sch.R, c->stream));
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
     HIPCHK(c, hipEventSynchronize(c->ev[2]));
     float a = 0.f, b = 0.f;
@@ -801,8 +906,8 @@ static int enqueue_sweeps(ll_batch *b, uint32_t iters) {
   for (uint32_t i = 0; i < iters; i++)
     for (auto &grp : b->groups)
       if (grp.sch.path == 1) {
-        BCHK(b, launch_square(grp.sch.sk, grp.d_args, (uint32_t)grp.members.size(), grp.sch.W, b->stream));
-        BCHK(b, launch_carry(grp.sch.ck, grp.d_args, (uint32_t)grp.members.size(), grp.sch.H, b->stream));
+        BCHK(b, launch_square(grp.sch.sk, grp.d_args, (uint32_t)grp.members.size(), grp.sch.W, grp.sch.H, b->stream));
+        BCHK(b, launch_carry(grp.sch.ck, grp.d_args, (uint32_t)grp.members.size(), grp.sch.W, grp.sch.H, grp.sch.R, b->stream));
       }
   return LL_OK;
 }
@@ -865,7 +970,7 @@ int ll_batch_create(int32_t device, uint32_t count, const ll_batch_test *tests, 
     BatchGroup *grp = nullptr;
     for (auto &g : b->groups)
       if (g.n == c->n && g.sch.path == c->sch.path && g.sch.W == c->sch.W && g.sch.H == c->sch.H &&
-          g.sch.R == c->sch.R && g.sch.rk == c->sch.rk)
+          g.sch.R == c->sch.R && g.sch.rk == c->sch.rk && g.sch.generic == c->sch.generic)
         grp = &g;
     if (!grp) {
       b->groups.emplace_back();
@@ -1024,6 +1129,11 @@ int ll_fft_schedules(uint32_t n, ll_schedule *out, uint32_t capacity) {
     bool dup = false;
     for (const auto &e : v) dup |= e.path == 1 && e.width == ck->w && e.lines == ck->r;
     if (!dup) v.push_back({n, 1, ck->w, H, ck->r, 0.f});
+  }
+  if (v.empty()) {  // lengths only the generic kernels run
+    Schedule g;
+    memset(&g, 0, sizeof g);
+    if (pick_generic(M, 0, nullptr, &g)) v.push_back({n, 1, g.W, g.H, g.R, 0.f});
   }
   if (out) {
     if (capacity < v.size()) return LL_ERR_ARG;

@@ -1,0 +1,401 @@
+// Generic two-sweep kernels: the LL iteration for any W x H x R schedule
+// whose sizes are known only at run time, one instantiation for all of
+// them.  They cover the FFT lengths of the reference's list
+// (CUDALucas.cu:955-971) that the specialised tables (ll_kernels.hip,
+// ll_carry_*.hip) have no template for: lines and columns with repeated or
+// several odd factors (3136K = 2^15 * 49 complex points: W = 784) and
+// lines beyond 2048 points (lengths above 16M words).
+//
+// The transforms run in place in dynamic LDS with a runtime radix plan
+// (GenLine): forward passes decimate in frequency (natural order in,
+// digit-reversed out), inverse passes decimate in time (digit-reversed in,
+// natural out), so no pass needs a second buffer; the digit reversal is
+// undone where data enters or leaves LDS (perm / iperm tables).  Twiddles
+// come from one table of L roots exp(-2 pi i e/L) per plan.  The HBM layout
+// of the signal, the four-step twiddle, the pairing squaring, the exact
+// FP64 carries and the epoch-tagged carry hand-off are those of the
+// specialised kernels (ll_sweeps.h), so both produce the same integers.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ll_sweeps.h"
+
+namespace llamd {
+
+namespace {
+
+constexpr int kGenThreads = 256;
+
+// One butterfly of a pass: forward (DIF) reads R values at stride s, takes
+// their length-R DFT and multiplies output k by w_L^(k j tstep); inverse
+// (DIT) multiplies input k by the conjugate, then takes the inverse DFT.
+template <int R, bool INV>
+__device__ __forceinline__ void gen_bfly(cplx *buf, int base, int s, int jt, const cplx *__restrict__ roots) {
+  cplx v[R];
+#pragma unroll
+  for (int i = 0; i < R; i++) v[i] = buf[lpad(base + i * s)];
+  if constexpr (INV) {
+    if (jt) {
+#pragma unroll
+      for (int k = 1; k < R; k++) v[k] = cmulc(v[k], gload(roots + k * jt));
+    }
+    dft<R, true>(v);
+  } else {
+    dft<R, false>(v);
+    if (jt) {
+#pragma unroll
+      for (int k = 1; k < R; k++) v[k] = cmul(v[k], gload(roots + k * jt));
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < R; i++) buf[lpad(base + i * s)] = v[i];
+}
+
+// One pass of radix R and stride s over NB transforms at lds + b * LP.
+template <bool INV>
+__device__ void gen_pass(cplx *lds, int LP, int NB, const GenLine &g, int R, int s) {
+  const int L = (int)g.L;
+  const int Q = L / R;
+  const int tstep = L / (R * s);
+  for (int u = threadIdx.x; u < NB * Q; u += blockDim.x) {
+    const int b = u / Q, v = u - b * Q;
+    const int blk = v / s, j = v - blk * s;
+    const int base = blk * R * s + j;
+    cplx *buf = lds + b * LP;
+    const int jt = j * tstep;
+    switch (R) {
+      case 2: gen_bfly<2, INV>(buf, base, s, jt, g.roots); break;
+      case 3: gen_bfly<3, INV>(buf, base, s, jt, g.roots); break;
+      case 4: gen_bfly<4, INV>(buf, base, s, jt, g.roots); break;
+      case 5: gen_bfly<5, INV>(buf, base, s, jt, g.roots); break;
+      case 7: gen_bfly<7, INV>(buf, base, s, jt, g.roots); break;
+      case 8: gen_bfly<8, INV>(buf, base, s, jt, g.roots); break;
+      default: gen_bfly<16, INV>(buf, base, s, jt, g.roots); break;
+    }
+  }
+}
+
+// Whole transforms of the NB lines; callers synchronise before.
+template <bool INV>
+__device__ void gen_fft(cplx *lds, int LP, int NB, const GenLine &g) {
+  int stride[kGenMaxPasses];
+  int span = (int)g.L;
+  for (int p = 0; p < (int)g.np; p++) {
+    span /= g.radix[p];
+    stride[p] = span;
+  }
+  if constexpr (!INV) {
+    for (int p = 0; p < (int)g.np; p++) {
+      gen_pass<false>(lds, LP, NB, g, g.radix[p], stride[p]);
+      __syncthreads();
+    }
+  } else {
+    for (int p = (int)g.np - 1; p >= 0; p--) {
+      gen_pass<true>(lds, LP, NB, g, g.radix[p], stride[p]);
+      __syncthreads();
+    }
+  }
+}
+
+__device__ __forceinline__ float gen_block_max(float v, float *red) {
+  v = wave_max(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  float m = 0.f;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) m = fmaxf(m, red[w]);
+  return m;  // valid in thread 0
+}
+
+}  // namespace
+
+// --------------------------------------------------------- square sweep
+// W / 2 workgroups; workgroup g owns columns g and W - g (0 and W / 2 for
+// g = 0), as square_sweep.
+__global__ __launch_bounds__(kGenThreads) void square_generic(const PlanArgs *__restrict__ args) {
+  extern __shared__ cplx dyn[];
+  const PlanArgs &A = args[blockIdx.y];
+  const bool finished = A.remaining && *A.remaining == 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    DevState &st = *A.st;
+    if (finished) {
+      st.run = 0;
+    } else {
+      const Minus2 m2 = minus2_position(st.offset, A.p, A.n);
+      st.m2_digit = m2.digit;
+      st.m2_value = m2.value;
+      st.offset = m2.new_offset;
+      A.err[st.epoch & A.err_mask] = 0u;
+      st.epoch += 1;
+      st.run = 1;
+    }
+  }
+  if (finished) return;
+  const int H = (int)A.H, W = (int)A.W, LP = lpitch(H);
+  const uint32_t g = blockIdx.x;
+  const uint32_t cols[2] = {g, g == 0 ? (uint32_t)W / 2 : (uint32_t)W - g};
+  const uint32_t rs = A.rshift, rmask = (1u << rs) - 1;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < 2 * H; i += blockDim.x) {
+    const int which = i >= H, b = i - which * H;
+    const uint32_t c = cols[which];
+    const size_t off = (((size_t)((uint32_t)b >> rs) * W + c) << rs) | ((uint32_t)b & rmask);
+    dyn[which * LP + lpad(b)] = cmul(gload_stream(A.x + off), twiddle_m(A.twM, c * (uint32_t)b));
+  }
+  __syncthreads();
+  gen_fft<false>(dyn, LP, 2, A.gH);
+  const uint32_t *ip = A.gH.iperm;
+  if (g != 0) {
+    cplx *c0 = dyn, *c1 = dyn + LP;
+    for (int k2 = tid; k2 < H; k2 += blockDim.x) {
+      const int pa = lpad((int)ip[k2]), pb = lpad((int)ip[H - 1 - k2]);
+      cplx a = c0[pa], b = c1[pb];
+      pair_square(a, b, twiddle_m(A.twM, g + (uint32_t)W * (uint32_t)k2));
+      c0[pa] = a;
+      c1[pb] = b;
+    }
+  } else {
+    // column 0: k2 <-> H - k2 ; column W/2: k2 <-> H - 1 - k2
+    for (int k2 = tid; k2 <= H / 2; k2 += blockDim.x) {
+      const int k2b = (H - k2) % H;
+      const int pa = lpad((int)ip[k2]), pb = lpad((int)ip[k2b]);
+      cplx a = dyn[pa], b = dyn[pb];
+      pair_square(a, b, twiddle_m(A.twM, (uint32_t)W * (uint32_t)k2));
+      dyn[pa] = a;
+      if (k2b != k2) dyn[pb] = b;
+    }
+    cplx *c1 = dyn + LP;
+    for (int k2 = tid; k2 < (H + 1) / 2; k2 += blockDim.x) {
+      const int k2b = H - 1 - k2;
+      const int pa = lpad((int)ip[k2]), pb = lpad((int)ip[k2b]);
+      cplx a = c1[pa], b = c1[pb];
+      pair_square(a, b, twiddle_m(A.twM, (uint32_t)W / 2 + (uint32_t)W * (uint32_t)k2));
+      c1[pa] = a;
+      if (k2b != k2) c1[pb] = b;
+    }
+  }
+  __syncthreads();
+  gen_fft<true>(dyn, LP, 2, A.gH);
+  const StreamOut out(A.x, A.n / 2);
+  for (int i = tid; i < 2 * H; i += blockDim.x) {
+    const int which = i >= H, b = i - which * H;
+    const uint32_t c = cols[which];
+    const uint32_t off = (((uint32_t)b >> rs) * (uint32_t)W + c) << rs | ((uint32_t)b & rmask);
+    out.store(off, cmulc(dyn[which * LP + lpad(b)], twiddle_m(A.twM, c * (uint32_t)b)));
+  }
+}
+
+// ---------------------------------------------------------- carry sweep
+// H / R workgroups, workgroup k owning block k of R lines, as carry_sweep:
+// inverse line transforms, unweight, round, exact carries along each
+// column's 2R words, hand-off of the chain's carry-out to the next block
+// (epoch-tagged shuttle), carry-in through the whole chain, -2, reweight,
+// forward line transforms.  Balanced digits wait in LDS for the carry-in.
+__global__ __launch_bounds__(kGenThreads) void carry_generic(const PlanArgs *__restrict__ args) {
+  extern __shared__ cplx dyn[];
+  const PlanArgs &A = args[blockIdx.y];
+  if (!A.st->run) return;  // a finished batched test
+  const int W = (int)A.W, R = 1 << A.rshift, LP = lpitch(W);
+  const uint32_t H = A.H, G = H / (uint32_t)R, blk = blockIdx.x, b0 = blk * (uint32_t)R;
+  cplx *lds = dyn;
+  double *wb = (double *)(lds + R * LP);
+  double *iwb = wb + 2 * R;
+  int *fb = (int *)(iwb + 2 * R);
+  float *red = (float *)(fb + 2 * R);
+  const int tid = threadIdx.x;
+  const uint32_t *ip = A.gW.iperm;
+  const int rs = (int)A.rshift;
+  for (int e = tid; e < R * W; e += blockDim.x) {
+    const int r = e & (R - 1), a = e >> rs;
+    lds[r * LP + lpad((int)ip[a])] = gload_stream(A.x + (size_t)b0 * W + e);
+  }
+  if (tid < 2 * R) {
+    const uint32_t i = 2 * b0 + tid;
+    fb[tid] = gload(A.fB + i);
+    wb[tid] = gload(A.wB + i);
+    iwb[tid] = gload(A.iwB + i) * A.scale;
+  }
+  const uint32_t epoch = A.st->epoch;
+  const uint32_t m2_digit = A.st->m2_digit;
+  const double dmod = (double)A.st->m2_value;
+  __syncthreads();
+  gen_fft<true>(lds, LP, R, A.gW);
+
+  const int q = (int)A.q;
+  const int n = (int)A.n, bb = (int)A.b;
+  double emax = 0.0;
+  // round, subtract, carry along each column's chain; publish the carry-out
+  for (int a = tid; a < W; a += blockDim.x) {
+    const int fa = gload(A.fA + a);
+    const double iwa = gload(A.iwA + a);
+    const uint32_t m2i = m2_digit - 2u * ((uint32_t)a * H + b0);
+    double carry = 0.0;
+    for (int r = 0; r < R; r++) {
+      const cplx v = lds[r * LP + lpad(a)];
+      double d[2];
+      for (int e = 0; e < 2; e++) {
+        const int i = 2 * r + e;
+        const int f0 = fa + fb[i];
+        const int wrap = f0 >= n;
+        const int big = f0 - (wrap ? n : 0) < bb;
+        const double t = ldexp((e ? v.y : v.x) * (iwa * iwb[i]), wrap);
+        double rt = rint(t);
+        emax = fmax(emax, fabs(t - rt));
+        if ((uint32_t)i == m2i) rt -= dmod;
+        d[e] = balance_f64(rt + carry, q + big, carry);
+      }
+      lds[r * LP + lpad(a)] = {d[0], d[1]};
+    }
+    const bool fits = fabs(carry) < 2147483648.0;
+    if (!fits) emax = 2.0;
+    const int32_t c32 = fits ? (int32_t)carry : 0;
+    const uint64_t word = ((uint64_t)epoch << 32) | (uint32_t)c32;
+    __hip_atomic_store(&A.shuttle[(size_t)blk * W + a], word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  // carry-in from the previous block (block 0: the last block, one column
+  // to the left), through the whole chain, then reweight
+  for (int a = tid; a < W; a += blockDim.x) {
+    const size_t src = blk == 0 ? (size_t)(G - 1) * W + (a + W - 1) % W : (size_t)(blk - 1) * W + a;
+    uint64_t word;
+    uint32_t spins = 0;
+    for (;;) {
+      word = __hip_atomic_load(&A.shuttle[src], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if ((uint32_t)(word >> 32) == epoch) break;
+      if (++spins > kSpinLimit) {
+        emax = 3.0;
+        word = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    const int fa = gload(A.fA + a);
+    const double wa = gload(A.wA + a);
+    double cc = (double)(int32_t)(uint32_t)word;
+    for (int r = 0; r < R; r++) {
+      const cplx dv = lds[r * LP + lpad(a)];
+      double x[2];
+      for (int e = 0; e < 2; e++) {
+        const int i = 2 * r + e;
+        const int f0 = fa + fb[i];
+        const int wrap = f0 >= n;
+        const int width = q + (f0 - (wrap ? n : 0) < bb);
+        double d = balance_f64((e ? dv.y : dv.x) + cc, width, cc);
+        if (i == 2 * R - 1) d += ldexp(cc, width);
+        x[e] = ldexp(d * (wa * wb[i]), -wrap);
+      }
+      lds[r * LP + lpad(a)] = {x[0], x[1]};
+    }
+  }
+  __syncthreads();
+  gen_fft<false>(lds, LP, R, A.gW);
+  const StreamOut out(A.x, A.n / 2);
+  for (int e = tid; e < R * W; e += blockDim.x) {
+    const int r = e & (R - 1), a = e >> rs;
+    out.store(b0 * (uint32_t)W + e, lds[r * LP + lpad((int)ip[a])]);
+  }
+  const float m = gen_block_max((float)emax, red);
+  if (tid == 0) {
+    atomicMax(&A.err[(epoch - 1) & A.err_mask], __float_as_uint(m));
+    if (blk == 0 && A.remaining) *A.remaining -= 1;
+  }
+}
+
+// ------------------------------------------------------ load / unload (HBM)
+// load: balanced digits (natural order) -> weighted, line-transformed
+// signal, the state the square sweep expects.
+__global__ __launch_bounds__(kGenThreads) void load_generic(const PlanArgs *__restrict__ args) {
+  extern __shared__ cplx dyn[];
+  const PlanArgs &A = args[blockIdx.y];
+  const int W = (int)A.W, R = 1 << A.rshift, LP = lpitch(W), rs = (int)A.rshift;
+  const uint32_t H = A.H, b0 = blockIdx.x * (uint32_t)R;
+  const int tid = threadIdx.x;
+  for (int a = tid; a < W; a += blockDim.x) {
+    const int fa = A.fA[a];
+    const double wa = A.wA[a], iwa = A.iwA[a];
+    for (int r = 0; r < R; r++) {
+      const uint32_t j = 2u * ((uint32_t)a * H + b0 + r);
+      const WordWeight we = word_weight(A, fa, wa, iwa, 2 * (b0 + r));
+      const WordWeight wo = word_weight(A, fa, wa, iwa, 2 * (b0 + r) + 1);
+      dyn[r * LP + lpad(a)] = {(double)A.digits[j] * we.w, (double)A.digits[j + 1] * wo.w};
+    }
+  }
+  __syncthreads();
+  gen_fft<false>(dyn, LP, R, A.gW);
+  const uint32_t *ip = A.gW.iperm;
+  for (int e = tid; e < R * W; e += blockDim.x) {
+    const int r = e & (R - 1), a = e >> rs;
+    A.x[(size_t)b0 * W + e] = dyn[r * LP + lpad((int)ip[a])];
+  }
+}
+
+// unload: inverse of load; digits rounded, rounding error max into err[0].
+__global__ __launch_bounds__(kGenThreads) void unload_generic(const PlanArgs *__restrict__ args) {
+  extern __shared__ cplx dyn[];
+  const PlanArgs &A = args[blockIdx.y];
+  const int W = (int)A.W, R = 1 << A.rshift, LP = lpitch(W), rs = (int)A.rshift;
+  const uint32_t H = A.H, b0 = blockIdx.x * (uint32_t)R;
+  float *red = (float *)(dyn + R * LP);
+  const int tid = threadIdx.x;
+  const uint32_t *ip = A.gW.iperm;
+  for (int e = tid; e < R * W; e += blockDim.x) {
+    const int r = e & (R - 1), a = e >> rs;
+    dyn[r * LP + lpad((int)ip[a])] = A.x[(size_t)b0 * W + e];
+  }
+  __syncthreads();
+  gen_fft<true>(dyn, LP, R, A.gW);
+  float emax = 0.f;
+  const double scale = 1.0 / (double)W;
+  for (int a = tid; a < W; a += blockDim.x) {
+    const int fa = A.fA[a];
+    const double wa = A.wA[a], iwa = A.iwA[a];
+    for (int r = 0; r < R; r++) {
+      const uint32_t j = 2u * ((uint32_t)a * H + b0 + r);
+      const cplx v = dyn[r * LP + lpad(a)];
+      const WordWeight we = word_weight(A, fa, wa, iwa, 2 * (b0 + r));
+      const WordWeight wo = word_weight(A, fa, wa, iwa, 2 * (b0 + r) + 1);
+      const double t0 = v.x * we.iw * scale, t1 = v.y * wo.iw * scale;
+      const double r0 = rint(t0), r1 = rint(t1);
+      emax = fmaxf(emax, fmaxf((float)fabs(t0 - r0), (float)fabs(t1 - r1)));
+      A.digits[j] = (int32_t)r0;
+      A.digits[j + 1] = (int32_t)r1;
+    }
+  }
+  const float m = gen_block_max(emax, red);
+  if (tid == 0) atomicMax(&A.err[0], __float_as_uint(m));
+}
+
+// ------------------------------------------------------------- dispatch
+
+uint32_t square_generic_lds(uint32_t H) { return 2u * (uint32_t)lpitch((int)H) * (uint32_t)sizeof(cplx); }
+
+uint32_t carry_generic_lds(uint32_t W, uint32_t R) {
+  return R * (uint32_t)lpitch((int)W) * (uint32_t)sizeof(cplx) + 2 * R * (8 + 8 + 4) + (kGenThreads / 64) * 4;
+}
+
+static const SquareKernel k_square_generic = {0, (const void *)square_generic, kGenThreads, nullptr};
+static const CarryKernel k_carry_generic = {0, 0, (const void *)carry_generic, (const void *)load_generic,
+                                            (const void *)unload_generic, kGenThreads, nullptr};
+
+// The generic kernels take up to kGenMaxLds bytes of dynamic LDS, above
+// the default launch limit: raised once per process (and device set).
+static void allow_generic_lds() {
+  static bool done = false;
+  if (done) return;
+  const void *fns[] = {(const void *)square_generic, (const void *)carry_generic, (const void *)load_generic,
+                       (const void *)unload_generic};
+  for (const void *f : fns) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kGenMaxLds);
+  done = true;
+}
+
+const SquareKernel *generic_square() {
+  allow_generic_lds();
+  return &k_square_generic;
+}
+
+const CarryKernel *generic_carry() {
+  allow_generic_lds();
+  return &k_carry_generic;
+}
+
+}  // namespace llamd

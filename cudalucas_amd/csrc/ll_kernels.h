@@ -28,6 +28,21 @@ struct DevState {
   long long m2_value;  // 2^(bit position inside that digit)
 };
 
+// A runtime mixed-radix plan of one transform length L (the generic
+// kernels): passes of radix radix[0], radix[1], ... (each 2, 3, 4, 5, 7, 8
+// or 16) run in place, decimation in frequency forward (natural order in,
+// digit-reversed out) and in time inverse (digit-reversed in, natural out).
+// Position q of a forward result holds frequency perm[q]; frequency k sits
+// at position iperm[k].
+constexpr int kGenMaxPasses = 16;
+struct GenLine {
+  const cplx *roots;      // exp(-2 pi i e/L), e < L
+  const uint32_t *perm;   // position -> frequency
+  const uint32_t *iperm;  // frequency -> position
+  uint32_t L, np;
+  uint8_t radix[kGenMaxPasses];
+};
+
 struct PlanArgs {
   cplx *x;                 // M complex; element a*H + b at ((b/R) W + a) R + b mod R
   const double *qcW;       // cos(2 pi e/W), e <= W/4
@@ -49,6 +64,7 @@ struct PlanArgs {
   uint32_t rshift;         // log2(R): lines are tiled by R in memory
   uint32_t *remaining;     // iterations left (batched tests), or null
   uint32_t p, n, W, H, q, b;
+  GenLine gW, gH;          // plans of the generic kernels (unused otherwise)
 };
 
 struct ResidentDesc {
@@ -64,18 +80,28 @@ struct ResidentDesc {
 };
 
 struct SquareKernel {
-  uint32_t h;
+  uint32_t h;         // 0: the generic kernel (any column length, runtime plan)
   const void *fn;
   uint32_t threads;
   const void *table;  // builds the per-pass twiddles of the column plan
 };
 
 struct CarryKernel {
-  uint32_t w, r;
+  uint32_t w, r;      // 0, 0: the generic kernel (any line length and R)
   const void *carry, *load, *unload;
   uint32_t threads;
   const void *table;  // builds the line pass table ptW (null: not used)
 };
+
+// The generic square / carry kernels (ll_generic.hip): one instantiation
+// for every W x H x R the specialised tables lack.  Dynamic LDS bytes of a
+// launch are square_generic_lds(H) and carry_generic_lds(W, R); lengths
+// whose LDS exceeds kGenMaxLds have no generic schedule.
+constexpr uint32_t kGenMaxLds = 160 * 1024;
+const SquareKernel *generic_square();
+const CarryKernel *generic_carry();
+uint32_t square_generic_lds(uint32_t H);
+uint32_t carry_generic_lds(uint32_t W, uint32_t R);
 
 struct ResidentKernel {
   uint32_t m;
@@ -101,14 +127,16 @@ const ResidentKernel *find_resident(uint32_t M);
 
 // Sweep launches over `tests` tests of one shape: d_args is a device array of
 // their PlanArgs, blockIdx.y selects the test.
+// W, H, R: the schedule's line length, column length and lines per carry
+// block (the generic kernels take them at run time).
 hipError_t launch_square(const SquareKernel *k, const PlanArgs *d_args, uint32_t tests, uint32_t W,
-                         hipStream_t s);
-hipError_t launch_carry(const CarryKernel *k, const PlanArgs *d_args, uint32_t tests, uint32_t H,
-                        hipStream_t s);
-hipError_t launch_load(const CarryKernel *k, const PlanArgs *d_args, uint32_t tests, uint32_t H,
-                       hipStream_t s);
-hipError_t launch_unload(const CarryKernel *k, const PlanArgs *d_args, uint32_t tests, uint32_t H,
-                         hipStream_t s);
+                         uint32_t H, hipStream_t s);
+hipError_t launch_carry(const CarryKernel *k, const PlanArgs *d_args, uint32_t tests, uint32_t W,
+                        uint32_t H, uint32_t R, hipStream_t s);
+hipError_t launch_load(const CarryKernel *k, const PlanArgs *d_args, uint32_t tests, uint32_t W,
+                       uint32_t H, uint32_t R, hipStream_t s);
+hipError_t launch_unload(const CarryKernel *k, const PlanArgs *d_args, uint32_t tests, uint32_t W,
+                         uint32_t H, uint32_t R, hipStream_t s);
 hipError_t launch_resident(const ResidentKernel *k, const ResidentDesc *d_descs, uint32_t count,
                            hipStream_t s);
 hipError_t launch_resident_unload(const ResidentDesc *d_descs, uint32_t count, int32_t *digits,

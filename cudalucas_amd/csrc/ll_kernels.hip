@@ -243,31 +243,35 @@ const ResidentKernel *find_resident(uint32_t M) {
 }
 
 hipError_t launch_square(const SquareKernel *k, const PlanArgs *d_args, uint32_t tests, uint32_t W,
-                         hipStream_t s) {
+                         uint32_t H, hipStream_t s) {
   const PlanArgs *a = d_args;
   void *args[] = {&a};
-  return hipLaunchKernel(k->fn, dim3(W / 2, tests), dim3(k->threads), args, 0, s);
+  const uint32_t lds = k->h ? 0 : square_generic_lds(H);
+  return hipLaunchKernel(k->fn, dim3(W / 2, tests), dim3(k->threads), args, lds, s);
 }
 
-hipError_t launch_carry(const CarryKernel *k, const PlanArgs *d_args, uint32_t tests, uint32_t H,
-                        hipStream_t s) {
+hipError_t launch_carry(const CarryKernel *k, const PlanArgs *d_args, uint32_t tests, uint32_t W,
+                        uint32_t H, uint32_t R, hipStream_t s) {
   const PlanArgs *a = d_args;
   void *args[] = {&a};
-  return hipLaunchKernel(k->carry, dim3(H / k->r, tests), dim3(k->threads), args, 0, s);
+  const uint32_t lds = k->w ? 0 : carry_generic_lds(W, R);
+  return hipLaunchKernel(k->carry, dim3(H / R, tests), dim3(k->threads), args, lds, s);
 }
 
-hipError_t launch_load(const CarryKernel *k, const PlanArgs *d_args, uint32_t tests, uint32_t H,
-                       hipStream_t s) {
+hipError_t launch_load(const CarryKernel *k, const PlanArgs *d_args, uint32_t tests, uint32_t W,
+                       uint32_t H, uint32_t R, hipStream_t s) {
   const PlanArgs *a = d_args;
   void *args[] = {&a};
-  return hipLaunchKernel(k->load, dim3(H / k->r, tests), dim3(k->threads), args, 0, s);
+  const uint32_t lds = k->w ? 0 : carry_generic_lds(W, R);
+  return hipLaunchKernel(k->load, dim3(H / R, tests), dim3(k->threads), args, lds, s);
 }
 
-hipError_t launch_unload(const CarryKernel *k, const PlanArgs *d_args, uint32_t tests, uint32_t H,
-                         hipStream_t s) {
+hipError_t launch_unload(const CarryKernel *k, const PlanArgs *d_args, uint32_t tests, uint32_t W,
+                         uint32_t H, uint32_t R, hipStream_t s) {
   const PlanArgs *a = d_args;
   void *args[] = {&a};
-  return hipLaunchKernel(k->unload, dim3(H / k->r, tests), dim3(k->threads), args, 0, s);
+  const uint32_t lds = k->w ? 0 : carry_generic_lds(W, R);
+  return hipLaunchKernel(k->unload, dim3(H / R, tests), dim3(k->threads), args, lds, s);
 }
 
 hipError_t launch_resident(const ResidentKernel *k, const ResidentDesc *d_descs, uint32_t count,

@@ -53,6 +53,8 @@ typedef struct ll_options {
 
 #define LL_OPT_FORCE_SWEEPS 1u /* use the two-sweep path even if LDS-resident fits */
 #define LL_OPT_NO_GRAPH 2u     /* launch kernels one by one (no hipGraph replay) */
+#define LL_OPT_GENERIC 4u      /* two sweeps on the runtime-plan (generic) kernels, which
+                                  every length without a specialised schedule uses */
 
 typedef struct ll_info {
   uint32_t p, n;

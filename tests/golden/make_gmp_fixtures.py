@@ -9,7 +9,7 @@ computed once here and committed as data.
 
 Jobs: the smooth-length sweep (iterations 200 and 2000), the large single
 exponent (every 100 iterations up to 1000), the two-sweep schedule cases,
-one length-coverage case per default FFT length above 1M words (the
+one length-coverage case per default FFT length above 16K words (the
 densest exponent the reference's policy assigns to it, workloads.densest_prime),
 and the learned selector's 512 evaluation primes after its 1000-iteration
 round-off test.
@@ -49,7 +49,7 @@ def entry(p, iters, words):
 
 # default FFT lengths (words) of the reference's list above this get a
 # length-coverage case
-LENGTH_CASES_ABOVE = 1 << 20
+LENGTH_CASES_ABOVE = 16384
 # iterations of the selector's round-off test (cudalucas_amd/selector.py CHECK_ITERS)
 SELECTOR_CHECK_ITERS = 1000
 

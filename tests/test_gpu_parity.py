@@ -262,14 +262,14 @@ def test_short_line_lengths_vs_gmp(nk):
 # ---------------------------------------------------- length coverage
 
 def _length_cases():
-    """Every default length this build supports above 1M words with its
-    committed GMP fixture: the densest exponent the reference's policy
-    assigns to the length."""
+    """Every default length this build supports above 16K words (the
+    two-sweep lengths) with its committed GMP fixture: the densest exponent
+    the reference's policy assigns to the length."""
     with open(os.path.join(GOLDEN, "gmp_residues.json")) as f:
         fx = json.load(f)
     out = []
     for n in O.default_fft_lengths():
-        if n <= (1 << 20) or not (os.path.exists(L.LIB_PATH) and L.fft_supported(n)):
+        if n <= 16384 or not (os.path.exists(L.LIB_PATH) and L.fft_supported(n)):
             continue
         p = W.densest_prime(n)
         out.append(pytest.param(p, n, W.length_check_iters(p), id="%dK" % (n // 1024),
@@ -321,3 +321,39 @@ def test_single_iteration_call_is_cheap():
           % (np.median(walls) * 1e6, np.median(gpus) * 1e6, wall20 / 20 * 1e6, gpu20 / 20 * 1e6))
     assert np.median(walls) < 1e-4
     assert wall20 < 1.15 * gpu20 + 1e-4
+
+
+# ------------------------------------------------------- generic kernels
+
+GENERIC_CASES = [
+    # (p, n, W, H, R): shapes the specialised kernels also run, forced onto
+    # the runtime-plan kernels, and shapes only they run
+    (2553659, 131072, 256, 256, 4),
+    (4723783, 262144, 512, 256, 8),
+    (9914521, 524288, 1024, 256, 2),
+    (19535569, 1048576, 1024, 512, 8),
+    (29097221, 1572864, 768, 1024, 8),   # 3 * 2^18
+    (4723783, 262144, 256, 512, 1),      # one line per carry block
+    (3021377, 165888, 36, 2304, 8),      # 162K: W = 9 * 4, H = 9 * 256
+    (4723783, 264600, 210, 630, 2),      # 2^3 3^3 5^2 7^2 / 2: every radix
+]
+
+
+@pytest.mark.parametrize("p,n,w,h,r", GENERIC_CASES)
+def test_generic_kernels_vs_gmp(p, n, w, h, r, gmp_fixtures):
+    iters = 200
+    announce("generic p=%d n=%d W=%d H=%d R=%d" % (p, n, w, h, r))
+    with L.LLContext(p, n, width=w, height=h, lines=r, flags=L.LL_OPT_GENERIC) as c:
+        info = c.info()
+        assert (info["width"], info["height"], info["lines"]) == (w, h, r)
+        c.load_initial(4242)
+        w0, _ = c.residue()
+        m = c.iterate(iters)
+        words, res = c.residue()
+    assert (w0 == O.gmp_ll(p, 0)).all()  # load / unload round trip
+    fx = gmp_fixtures.get((p, iters))
+    if fx is not None:
+        assert "0x%016x" % res == fx["res64"] and sha(words) == fx["sha256"]
+    else:
+        assert (words == O.gmp_ll(p, iters)).all()
+    assert m < ROUNDOFF_LIMIT
