@@ -41,7 +41,7 @@ bool seven_smooth(uint32_t x) {
 }
 
 // Radix passes of the generic plan of L: the odd prime factors, then the
-// power of two in radix-16 passes with an 8, 4 or 2 remainder first.
+// power of two in radix-8 passes with a 4 or 2 remainder first.
 // Empty if L is not 7-smooth or needs more than kGenMaxPasses passes.
 std::vector<uint32_t> generic_radices(uint32_t L) {
   std::vector<uint32_t> r;
@@ -57,8 +57,8 @@ std::vector<uint32_t> generic_radices(uint32_t L) {
     k++;
   }
   if (x != 1) return {};
-  if (k % 4) r.push_back(1u << (k % 4));
-  for (int i = 0; i < k / 4; i++) r.push_back(16);
+  if (k % 3) r.push_back(1u << (k % 3));
+  for (int i = 0; i < k / 3; i++) r.push_back(8);
   if (r.size() > (size_t)kGenMaxPasses) return {};
   return r;
 }
@@ -66,21 +66,26 @@ std::vector<uint32_t> generic_radices(uint32_t L) {
 // The generic schedule of M complex points, if any: H | M columns of 64 to
 // 4800 points (two fit in LDS), W = M / H even lines, R a power of two
 // lines per carry block whose LDS fits; o may force W, H or R.  Preferred:
-// H near 2048, R = 8 (128-byte column runs) or 4, fewer line passes.
-bool pick_generic(uint32_t M, uint32_t want_r, const ll_options *o, Schedule *s) {
+// H near 2048, R = 8 (128-byte column runs) or 4, fewer line passes.  With
+// max_lds below kGenMaxLds (the batched engine: several workgroups per CU)
+// both sweeps' LDS stay under it and W near H / 2 is preferred, which
+// balances the two sweeps' workgroup counts.
+bool pick_generic(uint32_t M, uint32_t want_r, const ll_options *o, Schedule *s,
+                  uint32_t max_lds = kGenMaxLds) {
   double best = 1e30;
   bool found = false;
+  const bool balanced = max_lds < kGenMaxLds;
   for (uint32_t H = 64; H <= 4800; H += 2) {
-    if (M % H || !seven_smooth(H) || square_generic_lds(H) > kGenMaxLds) continue;
+    if (M % H || !seven_smooth(H) || square_generic_lds(H) > max_lds) continue;
     const uint32_t W = M / H;
     if (W % 2 || W < 8 || !seven_smooth(W)) continue;
     if (o && o->width && o->width != W) continue;
     if (o && o->height && o->height != H) continue;
     if (generic_radices(W).empty() || generic_radices(H).empty()) continue;
     for (uint32_t R = 16; R >= 1; R /= 2) {
-      if (H % R || carry_generic_lds(W, R) > kGenMaxLds) continue;
+      if (H % R || carry_generic_lds(W, R) > max_lds) continue;
       if (want_r && want_r != R) continue;
-      double cost = fabs(log2((double)H) - 11.0) * 2.0;
+      double cost = balanced ? fabs(log2((double)H / W) - 1.0) * 2.0 : fabs(log2((double)H) - 11.0) * 2.0;
       cost += R == 8 ? 0.0 : R == 4 ? 0.5 : R == 16 ? 0.7 : R == 2 ? 1.5 : 4.0;
       cost += 0.1 * (double)generic_radices(W).size();
       if (cost < best) {
@@ -826,6 +831,7 @@ int ll_debug_inject(ll_ctx *c, uint32_t kind, uint64_t after, float value) {
 namespace {
 
 constexpr uint32_t kBatchRing = 1024;  // iterations per batch step at most
+constexpr uint32_t kBatchLds = 40 * 1024;  // LDS per batched workgroup: 4 per CU
 
 struct BatchGroup {
   Schedule sch{};
@@ -853,6 +859,14 @@ struct ll_batch {
   std::vector<uint64_t> done;        // iterations run per test
   std::vector<float> max_err;        // running maximum per test
   std::vector<std::vector<float>> step_errs;  // per-iteration errors of the last step
+  // the single-launch iteration of the two-sweep tests
+  std::vector<uint32_t> sweep_tests;   // test indices of the two-sweep tests
+  PlanArgs *d_fargs = nullptr;         // their PlanArgs, in sweep_tests order
+  unsigned long long *d_sq_done = nullptr, *d_ca_done = nullptr;
+  BatchItem *d_items = nullptr;        // capacity: every item of every test
+  uint32_t item_count = 0;             // items of the tests still running
+  std::vector<bool> in_items;          // per sweep test: has items in d_items
+  uint32_t lds = 0;
   hipGraphExec_t graph = nullptr;
   double last_ms = 0.0;
   char msg[256] = {0};
@@ -896,19 +910,44 @@ static int balloc(ll_batch *b, T **ptr, size_t count) {
   return LL_OK;
 }
 
-// `iters` iterations of every two-sweep group on the batch stream.  The
-// groups' launches run one after another: a carry sweep waits on other
-// workgroups of its own launch, which is safe for one launch at a time
-// (each XCD dispatches its workgroups in order) but not for two carry
-// launches sharing the GPU, which can hold the slots each other's awaited
-// workgroups need.
+// `iters` iterations of every running two-sweep test on the batch stream:
+// one launch per iteration (batch_iteration) whose workgroups are every
+// test's square items, then every test's carry items.
 static int enqueue_sweeps(ll_batch *b, uint32_t iters) {
+  if (!b->item_count) return LL_OK;
   for (uint32_t i = 0; i < iters; i++)
-    for (auto &grp : b->groups)
-      if (grp.sch.path == 1) {
-        BCHK(b, launch_square(grp.sch.sk, grp.d_args, (uint32_t)grp.members.size(), grp.sch.W, grp.sch.H, b->stream));
-        BCHK(b, launch_carry(grp.sch.ck, grp.d_args, (uint32_t)grp.members.size(), grp.sch.W, grp.sch.H, grp.sch.R, b->stream));
+    BCHK(b, launch_batch_iteration(b->d_items, b->item_count, b->d_fargs, b->d_sq_done, b->d_ca_done, b->lds,
+                                   b->stream));
+  return LL_OK;
+}
+
+// The work items of the two-sweep tests that still run (all square items
+// first); rebuilt, with the captured graph dropped, when tests finish.
+static int build_items(ll_batch *b) {
+  std::vector<BatchItem> items;
+  std::vector<bool> in(b->sweep_tests.size(), false);
+  for (int carry = 0; carry < 2; carry++)
+    for (uint32_t k = 0; k < b->sweep_tests.size(); k++) {
+      const uint32_t t = b->sweep_tests[k];
+      if (b->remaining[t] == 0) continue;
+      in[k] = true;
+      const ll_ctx *c = b->tests[t];
+      const uint32_t n = carry ? c->sch.H / c->sch.R : c->sch.W / 2;
+      for (uint32_t blk = 0; blk < n; blk++) {
+        BatchItem it;
+        it.test = k;
+        it.carry = (uint32_t)carry;
+        it.block = blk;
+        items.push_back(it);
       }
+    }
+  if (in == b->in_items && b->item_count == items.size()) return LL_OK;
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  b->graph = nullptr;
+  b->in_items = in;
+  b->item_count = (uint32_t)items.size();
+  if (!items.empty())
+    BCHK(b, hipMemcpy(b->d_items, items.data(), items.size() * sizeof(BatchItem), hipMemcpyHostToDevice));
   return LL_OK;
 }
 
@@ -957,9 +996,22 @@ int ll_batch_create(int32_t device, uint32_t count, const ll_batch_test *tests, 
   memset(&o, 0, sizeof o);
   o.device = device;
   o.flags = flags & LL_OPT_FORCE_SWEEPS;
+  generic_kernels_init();
   for (uint32_t i = 0; i < count; i++) {
     ll_ctx *c = nullptr;
-    rc = ctx_create(tests[i].p, tests[i].n, &o, kBatchRing, false, &c);
+    // two-sweep tests run the generic kernels' bodies in the single-launch
+    // iteration, at a schedule whose LDS lets several workgroups share a CU
+    Schedule pick, gpick;
+    memset(&gpick, 0, sizeof gpick);
+    ll_options oi = o;
+    if (pick_schedule(tests[i].n, &o, &pick) && pick.path == 1 &&
+        pick_generic(tests[i].n / 2, 0, nullptr, &gpick, kBatchLds)) {
+      oi.flags |= LL_OPT_GENERIC;
+      oi.width = gpick.W;
+      oi.height = gpick.H;
+      oi.lines = gpick.R;
+    }
+    rc = ctx_create(tests[i].p, tests[i].n, &oi, kBatchRing, false, &c);
     if (c) b->tests.push_back(c);
     if (rc) return bfail(b, rc, "test %u (p=%u n=%u): %s", i, tests[i].p, tests[i].n, c ? c->msg : "");
     std::vector<int32_t> d(tests[i].n);
@@ -1003,6 +1055,31 @@ int ll_batch_create(int32_t device, uint32_t count, const ll_batch_test *tests, 
       if ((rc = balloc(b, &g.d_desc, k))) return rc;
     }
   }
+  // the single-launch iteration: PlanArgs of every two-sweep test, its
+  // counters, room for all items
+  std::vector<PlanArgs> fa;
+  size_t items = 0;
+  for (uint32_t i = 0; i < count; i++) {
+    const ll_ctx *c = b->tests[i];
+    if (c->sch.path != 1) continue;
+    PlanArgs a = c->args;
+    a.remaining = b->d_remaining + i;
+    a.err = b->d_ring + (size_t)i * kBatchRing;
+    a.err_mask = kBatchRing - 1;
+    fa.push_back(a);
+    b->sweep_tests.push_back(i);
+    items += c->sch.W / 2 + c->sch.H / c->sch.R;
+    b->lds = std::max(b->lds, std::max(square_generic_lds(c->sch.H), carry_generic_lds(c->sch.W, c->sch.R)));
+  }
+  if (!fa.empty()) {
+    if ((rc = balloc(b, &b->d_fargs, fa.size())) || (rc = balloc(b, &b->d_sq_done, fa.size())) ||
+        (rc = balloc(b, &b->d_ca_done, fa.size())) || (rc = balloc(b, &b->d_items, items)))
+      return rc;
+    BCHK(b, hipMemcpy(b->d_fargs, fa.data(), fa.size() * sizeof(PlanArgs), hipMemcpyHostToDevice));
+    BCHK(b, hipMemset(b->d_sq_done, 0, fa.size() * sizeof(unsigned long long)));
+    BCHK(b, hipMemset(b->d_ca_done, 0, fa.size() * sizeof(unsigned long long)));
+  }
+  if ((rc = build_items(b))) return rc;
   return LL_OK;
 }
 
@@ -1034,10 +1111,12 @@ int ll_batch_step(ll_batch *b, uint32_t iterations, float *step_max_err) {
                            hipMemcpyHostToDevice, b->stream2));
     BCHK(b, launch_resident(g.sch.rk, g.d_desc, (uint32_t)g.desc.size(), b->stream2));
   }
-  // two-sweep groups: square + carry per iteration, graph-replayed
-  bool any_sweeps = false;
-  for (auto &g : b->groups) any_sweeps |= g.sch.path == 1;
-  if (any_sweeps) {
+  // two-sweep tests: one launch per iteration, graph-replayed
+  {
+    int rc = build_items(b);
+    if (rc) return rc;
+  }
+  if (b->item_count) {
     uint32_t left = iterations;
     if (left >= kGraphIters) {
       int rc = batch_graph(b);

@@ -25,6 +25,9 @@ namespace llamd {
 namespace {
 
 constexpr int kGenThreads = 256;
+// at least 4 waves per SIMD (4 workgroups per CU where the LDS allows):
+// 128 registers per lane, so the largest radix is 8
+constexpr int kGenWavesPerEU = 4;
 
 // One butterfly of a pass: forward (DIF) reads R values at stride s, takes
 // their length-R DFT and multiplies output k by w_L^(k j tstep); inverse
@@ -53,7 +56,7 @@ __device__ __forceinline__ void gen_bfly(cplx *buf, int base, int s, int jt, con
 
 // One pass of radix R and stride s over NB transforms at lds + b * LP.
 template <bool INV>
-__device__ void gen_pass(cplx *lds, int LP, int NB, const GenLine &g, int R, int s) {
+__device__ __forceinline__ void gen_pass(cplx *lds, int LP, int NB, const GenLine &g, int R, int s) {
   const int L = (int)g.L;
   const int Q = L / R;
   const int tstep = L / (R * s);
@@ -69,33 +72,61 @@ __device__ void gen_pass(cplx *lds, int LP, int NB, const GenLine &g, int R, int
       case 4: gen_bfly<4, INV>(buf, base, s, jt, g.roots); break;
       case 5: gen_bfly<5, INV>(buf, base, s, jt, g.roots); break;
       case 7: gen_bfly<7, INV>(buf, base, s, jt, g.roots); break;
-      case 8: gen_bfly<8, INV>(buf, base, s, jt, g.roots); break;
-      default: gen_bfly<16, INV>(buf, base, s, jt, g.roots); break;
+      default: gen_bfly<8, INV>(buf, base, s, jt, g.roots); break;
     }
   }
 }
 
 // Whole transforms of the NB lines; callers synchronise before.
+// Pass p has stride s_p = L / (r_0 ... r_p): forward passes run p = 0, 1,
+// ...; inverse passes run backwards from s = 1, s_(p-1) = s_p r_p.
 template <bool INV>
-__device__ void gen_fft(cplx *lds, int LP, int NB, const GenLine &g) {
-  int stride[kGenMaxPasses];
-  int span = (int)g.L;
-  for (int p = 0; p < (int)g.np; p++) {
-    span /= g.radix[p];
-    stride[p] = span;
-  }
+__device__ __forceinline__ void gen_fft(cplx *lds, int LP, int NB, const GenLine &g) {
+  const int np = (int)g.np;
   if constexpr (!INV) {
-    for (int p = 0; p < (int)g.np; p++) {
-      gen_pass<false>(lds, LP, NB, g, g.radix[p], stride[p]);
+    int s = (int)g.L;
+    for (int p = 0; p < np; p++) {
+      const int r = g.radix[p];
+      s /= r;
+      gen_pass<false>(lds, LP, NB, g, r, s);
       __syncthreads();
     }
   } else {
-    for (int p = (int)g.np - 1; p >= 0; p--) {
-      gen_pass<true>(lds, LP, NB, g, g.radix[p], stride[p]);
+    int s = 1;
+    for (int p = np - 1; p >= 0; p--) {
+      const int r = g.radix[p];
+      gen_pass<true>(lds, LP, NB, g, r, s);
       __syncthreads();
+      s *= r;
     }
   }
 }
+
+// Loops over i = threadIdx.x + k * blockDim.x < n in chunks of kChunk, the
+// chunk's reads (ld) all issued before any result is used (st), so a
+// thread waits for memory once per chunk, not once per element.
+constexpr int kChunk = 4;
+template <typename LD, typename ST>
+__device__ __forceinline__ void chunked(int n, const LD &ld, const ST &st) {
+  const int T = blockDim.x;
+  for (int base = threadIdx.x; base < n; base += kChunk * T) {
+    decltype(ld(0)) v[kChunk];
+#pragma unroll
+    for (int k = 0; k < kChunk; k++)
+      if (base + k * T < n) v[k] = ld(base + k * T);
+#pragma unroll
+    for (int k = 0; k < kChunk; k++)
+      if (base + k * T < n) st(base + k * T, v[k]);
+  }
+}
+
+struct CplxPair {
+  cplx a, b;
+};
+struct CplxAt {
+  cplx v;
+  uint32_t pos;
+};
 
 __device__ __forceinline__ float gen_block_max(float v, float *red) {
   v = wave_max(v);
@@ -113,11 +144,9 @@ __device__ __forceinline__ float gen_block_max(float v, float *red) {
 // --------------------------------------------------------- square sweep
 // W / 2 workgroups; workgroup g owns columns g and W - g (0 and W / 2 for
 // g = 0), as square_sweep.
-__global__ __launch_bounds__(kGenThreads) void square_generic(const PlanArgs *__restrict__ args) {
-  extern __shared__ cplx dyn[];
-  const PlanArgs &A = args[blockIdx.y];
+__device__ __forceinline__ void square_generic_body(const PlanArgs &A, const uint32_t g, cplx *dyn) {
   const bool finished = A.remaining && *A.remaining == 0;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (g == 0 && threadIdx.x == 0) {  // column pair 0 opens the iteration
     DevState &st = *A.st;
     if (finished) {
       st.run = 0;
@@ -133,16 +162,21 @@ __global__ __launch_bounds__(kGenThreads) void square_generic(const PlanArgs *__
   }
   if (finished) return;
   const int H = (int)A.H, W = (int)A.W, LP = lpitch(H);
-  const uint32_t g = blockIdx.x;
   const uint32_t cols[2] = {g, g == 0 ? (uint32_t)W / 2 : (uint32_t)W - g};
   const uint32_t rs = A.rshift, rmask = (1u << rs) - 1;
   const int tid = threadIdx.x;
-  for (int i = tid; i < 2 * H; i += blockDim.x) {
-    const int which = i >= H, b = i - which * H;
-    const uint32_t c = cols[which];
-    const size_t off = (((size_t)((uint32_t)b >> rs) * W + c) << rs) | ((uint32_t)b & rmask);
-    dyn[which * LP + lpad(b)] = cmul(gload_stream(A.x + off), twiddle_m(A.twM, c * (uint32_t)b));
-  }
+  chunked(
+      2 * H,
+      [&](int i) {
+        const int which = i >= H, b = i - which * H;
+        const uint32_t c = cols[which];
+        const size_t off = (((size_t)((uint32_t)b >> rs) * W + c) << rs) | ((uint32_t)b & rmask);
+        return CplxPair{gload_stream(A.x + off), twiddle_m(A.twM, c * (uint32_t)b)};
+      },
+      [&](int i, const CplxPair &v) {
+        const int which = i >= H, b = i - which * H;
+        dyn[which * LP + lpad(b)] = cmul(v.a, v.b);
+      });
   __syncthreads();
   gen_fft<false>(dyn, LP, 2, A.gH);
   const uint32_t *ip = A.gH.iperm;
@@ -178,12 +212,23 @@ __global__ __launch_bounds__(kGenThreads) void square_generic(const PlanArgs *__
   __syncthreads();
   gen_fft<true>(dyn, LP, 2, A.gH);
   const StreamOut out(A.x, A.n / 2);
-  for (int i = tid; i < 2 * H; i += blockDim.x) {
-    const int which = i >= H, b = i - which * H;
-    const uint32_t c = cols[which];
-    const uint32_t off = (((uint32_t)b >> rs) * (uint32_t)W + c) << rs | ((uint32_t)b & rmask);
-    out.store(off, cmulc(dyn[which * LP + lpad(b)], twiddle_m(A.twM, c * (uint32_t)b)));
-  }
+  chunked(
+      2 * H,
+      [&](int i) {
+        const int which = i >= H, b = i - which * H;
+        return twiddle_m(A.twM, cols[which] * (uint32_t)b);
+      },
+      [&](int i, const cplx &w) {
+        const int which = i >= H, b = i - which * H;
+        const uint32_t c = cols[which];
+        const uint32_t off = (((uint32_t)b >> rs) * (uint32_t)W + c) << rs | ((uint32_t)b & rmask);
+        out.store(off, cmulc(dyn[which * LP + lpad(b)], w));
+      });
+}
+
+__global__ __launch_bounds__(kGenThreads, kGenWavesPerEU) void square_generic(const PlanArgs *__restrict__ args) {
+  extern __shared__ cplx dyn[];
+  square_generic_body(args[blockIdx.y], blockIdx.x, dyn);
 }
 
 // ---------------------------------------------------------- carry sweep
@@ -192,12 +237,10 @@ __global__ __launch_bounds__(kGenThreads) void square_generic(const PlanArgs *__
 // column's 2R words, hand-off of the chain's carry-out to the next block
 // (epoch-tagged shuttle), carry-in through the whole chain, -2, reweight,
 // forward line transforms.  Balanced digits wait in LDS for the carry-in.
-__global__ __launch_bounds__(kGenThreads) void carry_generic(const PlanArgs *__restrict__ args) {
-  extern __shared__ cplx dyn[];
-  const PlanArgs &A = args[blockIdx.y];
+__device__ __forceinline__ void carry_generic_body(const PlanArgs &A, const uint32_t blk, cplx *dyn) {
   if (!A.st->run) return;  // a finished batched test
   const int W = (int)A.W, R = 1 << A.rshift, LP = lpitch(W);
-  const uint32_t H = A.H, G = H / (uint32_t)R, blk = blockIdx.x, b0 = blk * (uint32_t)R;
+  const uint32_t H = A.H, G = H / (uint32_t)R, b0 = blk * (uint32_t)R;
   cplx *lds = dyn;
   double *wb = (double *)(lds + R * LP);
   double *iwb = wb + 2 * R;
@@ -206,10 +249,10 @@ __global__ __launch_bounds__(kGenThreads) void carry_generic(const PlanArgs *__r
   const int tid = threadIdx.x;
   const uint32_t *ip = A.gW.iperm;
   const int rs = (int)A.rshift;
-  for (int e = tid; e < R * W; e += blockDim.x) {
-    const int r = e & (R - 1), a = e >> rs;
-    lds[r * LP + lpad((int)ip[a])] = gload_stream(A.x + (size_t)b0 * W + e);
-  }
+  chunked(
+      R * W,
+      [&](int e) { return CplxAt{gload_stream(A.x + (size_t)b0 * W + e), gload(ip + (e >> rs))}; },
+      [&](int e, const CplxAt &v) { lds[(e & (R - 1)) * LP + lpad((int)v.pos)] = v.v; });
   if (tid < 2 * R) {
     const uint32_t i = 2 * b0 + tid;
     fb[tid] = gload(A.fB + i);
@@ -290,10 +333,9 @@ __global__ __launch_bounds__(kGenThreads) void carry_generic(const PlanArgs *__r
   __syncthreads();
   gen_fft<false>(lds, LP, R, A.gW);
   const StreamOut out(A.x, A.n / 2);
-  for (int e = tid; e < R * W; e += blockDim.x) {
-    const int r = e & (R - 1), a = e >> rs;
-    out.store(b0 * (uint32_t)W + e, lds[r * LP + lpad((int)ip[a])]);
-  }
+  chunked(
+      R * W, [&](int e) { return gload(ip + (e >> rs)); },
+      [&](int e, const uint32_t &q) { out.store(b0 * (uint32_t)W + e, lds[(e & (R - 1)) * LP + lpad((int)q)]); });
   const float m = gen_block_max((float)emax, red);
   if (tid == 0) {
     atomicMax(&A.err[(epoch - 1) & A.err_mask], __float_as_uint(m));
@@ -301,10 +343,77 @@ __global__ __launch_bounds__(kGenThreads) void carry_generic(const PlanArgs *__r
   }
 }
 
+__global__ __launch_bounds__(kGenThreads, kGenWavesPerEU) void carry_generic(const PlanArgs *__restrict__ args) {
+  extern __shared__ cplx dyn[];
+  carry_generic_body(args[blockIdx.y], blockIdx.x, dyn);
+}
+
+// ------------------------------------------------- batched: one launch
+// One LL iteration of every two-sweep test of a batch in a single launch.
+// Workgroup i runs item items[i]: first every test's square items, then
+// every test's carry items.  A test's carry items start once all its square
+// items of this iteration have finished (per-test counters: sq_done counts
+// square items ever finished, ca_done carry items; the carry items of
+// iteration e of a test wait for sq_done >= (e + 1) W / 2, e read from
+// ca_done, which this item has not yet raised).  Every wait is on items of
+// lower index (or on the last carry block's carries, published before it
+// waits), which each XCD dispatches first, as in the single-test carry
+// sweep; it holds while the tests are fewer than the workgroups the GPU
+// holds at once.
+__global__ __launch_bounds__(kGenThreads, kGenWavesPerEU) void batch_iteration(const BatchItem *__restrict__ items,
+                                                               const PlanArgs *__restrict__ args,
+                                                               unsigned long long *sq_done,
+                                                               unsigned long long *ca_done) {
+  extern __shared__ cplx dyn[];
+  __shared__ int s_go;
+  const BatchItem it = items[blockIdx.x];
+  const PlanArgs &A = args[it.test];
+  if (!it.carry) {
+    square_generic_body(A, it.block, dyn);
+    __syncthreads();
+    if (threadIdx.x == 0)
+      __hip_atomic_fetch_add(&sq_done[it.test], 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    return;
+  }
+  if (threadIdx.x == 0) {
+    const unsigned long long G = A.H >> A.rshift;
+    const unsigned long long e =
+        __hip_atomic_load(&ca_done[it.test], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) / G;
+    const unsigned long long want = (e + 1) * (A.W / 2);
+    uint32_t spins = 0;
+    int go = 1;
+    while (__hip_atomic_load(&sq_done[it.test], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < want) {
+      if (++spins > kSpinLimit) {  // never hang: the iteration reports failure
+        go = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    s_go = go;
+  }
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  if (s_go) {
+    carry_generic_body(A, it.block, dyn);
+  } else if (threadIdx.x == 0) {
+    atomicMax(&A.err[(A.st->epoch - 1) & A.err_mask], __float_as_uint(3.0f));
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    __hip_atomic_fetch_add(&ca_done[it.test], 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+hipError_t launch_batch_iteration(const BatchItem *items, uint32_t count, const PlanArgs *args,
+                                  unsigned long long *sq_done, unsigned long long *ca_done, uint32_t lds,
+                                  hipStream_t s) {
+  void *a[] = {&items, &args, &sq_done, &ca_done};
+  return hipLaunchKernel((const void *)batch_iteration, dim3(count), dim3(kGenThreads), a, lds, s);
+}
+
 // ------------------------------------------------------ load / unload (HBM)
 // load: balanced digits (natural order) -> weighted, line-transformed
 // signal, the state the square sweep expects.
-__global__ __launch_bounds__(kGenThreads) void load_generic(const PlanArgs *__restrict__ args) {
+__global__ __launch_bounds__(kGenThreads, kGenWavesPerEU) void load_generic(const PlanArgs *__restrict__ args) {
   extern __shared__ cplx dyn[];
   const PlanArgs &A = args[blockIdx.y];
   const int W = (int)A.W, R = 1 << A.rshift, LP = lpitch(W), rs = (int)A.rshift;
@@ -330,7 +439,7 @@ __global__ __launch_bounds__(kGenThreads) void load_generic(const PlanArgs *__re
 }
 
 // unload: inverse of load; digits rounded, rounding error max into err[0].
-__global__ __launch_bounds__(kGenThreads) void unload_generic(const PlanArgs *__restrict__ args) {
+__global__ __launch_bounds__(kGenThreads, kGenWavesPerEU) void unload_generic(const PlanArgs *__restrict__ args) {
   extern __shared__ cplx dyn[];
   const PlanArgs &A = args[blockIdx.y];
   const int W = (int)A.W, R = 1 << A.rshift, LP = lpitch(W), rs = (int)A.rshift;
@@ -383,10 +492,15 @@ static void allow_generic_lds() {
   static bool done = false;
   if (done) return;
   const void *fns[] = {(const void *)square_generic, (const void *)carry_generic, (const void *)load_generic,
-                       (const void *)unload_generic};
-  for (const void *f : fns) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kGenMaxLds);
+                       (const void *)unload_generic, (const void *)batch_iteration};
+  for (const void *f : fns) {
+    (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kGenMaxLds);
+    (void)hipGetLastError();  // a refusal must not linger as the next launch's error
+  }
   done = true;
 }
+
+void generic_kernels_init() { allow_generic_lds(); }
 
 const SquareKernel *generic_square() {
   allow_generic_lds();

@@ -97,11 +97,27 @@ struct CarryKernel {
 // for every W x H x R the specialised tables lack.  Dynamic LDS bytes of a
 // launch are square_generic_lds(H) and carry_generic_lds(W, R); lengths
 // whose LDS exceeds kGenMaxLds have no generic schedule.
-constexpr uint32_t kGenMaxLds = 160 * 1024;
+constexpr uint32_t kGenMaxLds = 160 * 1024 - 256;  // room for the kernels' own few bytes
 const SquareKernel *generic_square();
 const CarryKernel *generic_carry();
 uint32_t square_generic_lds(uint32_t H);
 uint32_t carry_generic_lds(uint32_t W, uint32_t R);
+void generic_kernels_init();
+
+// One work item of the batched single-launch iteration: square column pair
+// or carry block `block` of test `test` (an index into the launch's
+// PlanArgs array).
+struct BatchItem {
+  uint32_t test : 31;
+  uint32_t carry : 1;
+  uint32_t block;
+};
+// count items (all square items first), args per test, per-test counters
+// of finished square and carry items (zero at batch creation); lds: the
+// largest square_generic_lds / carry_generic_lds of the tests.
+hipError_t launch_batch_iteration(const BatchItem *items, uint32_t count, const PlanArgs *args,
+                                  unsigned long long *sq_done, unsigned long long *ca_done, uint32_t lds,
+                                  hipStream_t s);
 
 struct ResidentKernel {
   uint32_t m;
