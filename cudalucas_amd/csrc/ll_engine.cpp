@@ -335,19 +335,28 @@ static int build_sweeps(ll_ctx *c) {
   const uint32_t shift = std::min<uint32_t>(kTwoLevelShift, (uint32_t)log2((double)M));
   const uint32_t S = 1u << shift;
   const uint32_t nhi = (M + S - 1) / S;
-  std::vector<double> hi(2 * (size_t)nhi), lo(2 * (size_t)S);
-  for (uint32_t i = 0; i < nhi; i++) root_of_unity((uint64_t)i * S, M, &hi[2 * i], &hi[2 * i + 1]);
+  // coarse roots as (rounded value, rounding remainder) pairs
+  std::vector<double> hi(4 * (size_t)nhi), lo(2 * (size_t)S);
+  for (uint32_t i = 0; i < nhi; i++) {
+    long double re, im;
+    root_of_unity_ld((uint64_t)i * S, M, &re, &im);
+    hi[4 * i] = (double)re;
+    hi[4 * i + 1] = (double)im;
+    hi[4 * i + 2] = (double)(re - (long double)hi[4 * i]);
+    hi[4 * i + 3] = (double)(im - (long double)hi[4 * i + 1]);
+  }
   for (uint32_t i = 0; i < S; i++) root_minus_one(i, M, &lo[2 * i], &lo[2 * i + 1]);
   if ((rc = upload(c, (double **)&A.twM.hi, hi))) return rc;
   if ((rc = upload(c, (double **)&A.twM.lo, lo))) return rc;
   A.twM.shift = shift;
   A.twM.mask = S - 1;
   // factored weights: word 2H a + i has f = fA[a] + fB[i] (mod n)
-  std::vector<double> wA(W), iwA(W), wB(2 * H), iwB(2 * H);
+  std::vector<double> wA(W), iwA(W), iwAs(W), wB(2 * H), iwB(2 * H);
   std::vector<int32_t> fA(W), fB(2 * H);
   for (uint32_t a = 0; a < W; a++) {
     fA[a] = (int32_t)weight_exponent(p, n, (uint64_t)2 * H * a);
     weight_pair((uint32_t)fA[a], n, &wA[a], &iwA[a]);
+    iwAs[a] = inverse_weight_scaled((uint32_t)fA[a], n, 4ull * M);
   }
   for (uint32_t i = 0; i < 2 * H; i++) {
     fB[i] = (int32_t)weight_exponent(p, n, i);
@@ -355,6 +364,7 @@ static int build_sweeps(ll_ctx *c) {
   }
   if ((rc = upload(c, (double **)&A.wA, wA))) return rc;
   if ((rc = upload(c, (double **)&A.iwA, iwA))) return rc;
+  if ((rc = upload(c, (double **)&A.iwAs, iwAs))) return rc;
   if ((rc = upload(c, (int32_t **)&A.fA, fA))) return rc;
   if ((rc = upload(c, (double **)&A.wB, wB))) return rc;
   if ((rc = upload(c, (double **)&A.iwB, iwB))) return rc;

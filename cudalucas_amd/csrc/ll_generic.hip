@@ -257,7 +257,7 @@ __device__ __forceinline__ void carry_generic_body(const PlanArgs &A, const uint
     const uint32_t i = 2 * b0 + tid;
     fb[tid] = gload(A.fB + i);
     wb[tid] = gload(A.wB + i);
-    iwb[tid] = gload(A.iwB + i) * A.scale;
+    iwb[tid] = gload(A.iwB + i);
   }
   const uint32_t epoch = A.st->epoch;
   const uint32_t m2_digit = A.st->m2_digit;
@@ -271,7 +271,7 @@ __device__ __forceinline__ void carry_generic_body(const PlanArgs &A, const uint
   // round, subtract, carry along each column's chain; publish the carry-out
   for (int a = tid; a < W; a += blockDim.x) {
     const int fa = gload(A.fA + a);
-    const double iwa = gload(A.iwA + a);
+    const double iwa = gload(A.iwAs + a);
     const uint32_t m2i = m2_digit - 2u * ((uint32_t)a * H + b0);
     double carry = 0.0;
     for (int r = 0; r < R; r++) {
@@ -282,7 +282,7 @@ __device__ __forceinline__ void carry_generic_body(const PlanArgs &A, const uint
         const int f0 = fa + fb[i];
         const int wrap = f0 >= n;
         const int big = f0 - (wrap ? n : 0) < bb;
-        const double t = ldexp((e ? v.y : v.x) * (iwa * iwb[i]), wrap);
+        const double t = ldexp(mul3(e ? v.y : v.x, iwa, iwb[i]), wrap);
         double rt = rint(t);
         emax = fmax(emax, fabs(t - rt));
         if ((uint32_t)i == m2i) rt -= dmod;
@@ -325,7 +325,7 @@ __device__ __forceinline__ void carry_generic_body(const PlanArgs &A, const uint
         const int width = q + (f0 - (wrap ? n : 0) < bb);
         double d = balance_f64((e ? dv.y : dv.x) + cc, width, cc);
         if (i == 2 * R - 1) d += ldexp(cc, width);
-        x[e] = ldexp(d * (wa * wb[i]), -wrap);
+        x[e] = ldexp(mul3(d, wa, wb[i]), -wrap);
       }
       lds[r * LP + lpad(a)] = {x[0], x[1]};
     }

@@ -28,6 +28,13 @@ namespace llamd {
 // cos and sin of -2 pi e / L (the forward root), from an exact octant
 // reduction of the rational e/L and long double evaluation on [0, pi/4].
 void root_of_unity(uint64_t e, uint64_t L, double *re, double *im) {
+  long double r, i;
+  root_of_unity_ld(e, L, &r, &i);
+  *re = (double)r;
+  *im = (double)i;
+}
+
+void root_of_unity_ld(uint64_t e, uint64_t L, long double *re, long double *im) {
   e %= L;
   const uint64_t o = (8 * e) / L;
   const uint64_t r = 8 * e - o * L;
@@ -46,8 +53,8 @@ void root_of_unity(uint64_t e, uint64_t L, double *re, double *im) {
     case 6: c = st; s = -ct; break;
     default: c = cu; s = -su; break;
   }
-  *re = (double)c;
-  *im = (double)-s;
+  *re = c;
+  *im = -s;
 }
 
 // exp(-2 pi i e / L) - 1 for small angles, computed as
@@ -77,6 +84,10 @@ void quarter_cosines(uint32_t L, std::vector<double> &out) {
 uint32_t weight_exponent(uint32_t p, uint32_t n, uint64_t j) {
   const uint64_t b = p % n;
   return (uint32_t)((n - (b * j) % n) % n);
+}
+
+double inverse_weight_scaled(uint32_t f, uint32_t n, uint64_t four_m) {
+  return (double)(exp2l(-(long double)f / (long double)n) / (long double)four_m);
 }
 
 void weight_pair(uint32_t f, uint32_t n, double *w, double *iw) {

@@ -8,7 +8,10 @@
 
 namespace llamd {
 
-// exp(-2 pi i e/M) = hi[e >> shift] * (1 + lo[e & mask]), lo = (cos-1, sin)
+// exp(-2 pi i e/M) = h * (1 + lo[e & mask]), lo = (cos-1, sin), where the
+// coarse root h = hi[2k] + hi[2k + 1] (k = e >> shift) is kept as a rounded
+// value and its rounding remainder, so the product is within about half an
+// ulp of the exact root.
 struct TwoLevel {
   const cplx *hi;
   const cplx *lo;
@@ -51,6 +54,7 @@ struct PlanArgs {
   const cplx *ptW;         // line plan's per-pass twiddles (wave-local carry sweeps)
   TwoLevel twM;            // exp(-2 pi i e/M), e < M
   const double *wA, *iwA;  // 2^(+-fA[a]/n), a < W
+  const double *iwAs;      // 2^(-fA[a]/n) / (4 M), rounded once: the carry sweep's unweight
   const int32_t *fA;       // (-b * 2H a) mod n
   const double *wB, *iwB;  // 2^(+-fB[i]/n), i < 2H
   const int32_t *fB;       // (-b * i) mod n

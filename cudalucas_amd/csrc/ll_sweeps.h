@@ -50,9 +50,19 @@ constexpr uint32_t kSpinLimit = 1u << 22;
 // ------------------------------------------------------------------ helpers
 
 __device__ __forceinline__ cplx twiddle_m(const TwoLevel &t, uint32_t e) {
-  const cplx h = gload(t.hi + (e >> t.shift));
+  const uint32_t k = 2 * (e >> t.shift);
+  const cplx h = gload(t.hi + k), hr = gload(t.hi + k + 1);
   const cplx l = gload(t.lo + (e & t.mask));  // (cos - 1, sin) of a small angle
-  return {h.x + fma(h.x, l.x, -h.y * l.y), h.y + fma(h.x, l.y, h.y * l.x)};
+  // h + (remainder + h l): the small terms first, one rounding at the end
+  return {h.x + (hr.x + fma(h.x, l.x, -h.y * l.y)), h.y + (hr.y + fma(h.x, l.y, h.y * l.x))};
+}
+
+// x * (a * b) with the product a * b carried exactly (its rounding error
+// recovered by an fma): one rounding instead of two.
+__device__ __forceinline__ double mul3(double x, double a, double b) {
+  const double p = a * b;
+  const double pe = fma(a, b, -p);
+  return fma(x, p, x * pe);
 }
 
 // Balanced digit of value v in base 2^width and the carry out.
@@ -449,7 +459,7 @@ __global__ __launch_bounds__(T, (T >= 512 ? 1 : carry_groups_per_cu<W, R, T>()))
       const int a = tid + c * T;
       if (W % T == 0 || a < W) {
         fa_c[c] = gload(A.fA + a);
-        iwa_c[c] = gload(A.iwA + a);
+        iwa_c[c] = gload(A.iwAs + a);
         wa_c[c] = gload(A.wA + a);
       }
     }
@@ -457,7 +467,7 @@ __global__ __launch_bounds__(T, (T >= 512 ? 1 : carry_groups_per_cu<W, R, T>()))
       const uint32_t i = 2 * blk * R + tid;
       fb[tid] = gload(A.fB + i);
       wb[tid] = gload(A.wB + i);
-      iwb[tid] = gload(A.iwB + i) * A.scale;
+      iwb[tid] = gload(A.iwB + i);
     }
 #pragma unroll
     for (int s = 0; s < NE; s++) {
@@ -502,7 +512,7 @@ __global__ __launch_bounds__(T, (T >= 512 ? 1 : carry_groups_per_cu<W, R, T>()))
           const int wrap = f0 >= (int)A.n;
           const int big = f0 - (wrap ? (int)A.n : 0) < (int)A.b;
           if constexpr (PF) fl |= ((uint64_t)wrap | ((uint64_t)big << 1)) << (2 * i);
-          const double t = ldexp((e ? v.y : v.x) * (iwa * iwb[i]), wrap);
+          const double t = ldexp(mul3(e ? v.y : v.x, iwa, iwb[i]), wrap);
           double rt = rint(t);
           emax = fmax(emax, fabs(t - rt));
           if constexpr (decltype(with_m2)::value) rt -= (uint32_t)i == m2i ? dmod : 0.0;
@@ -531,7 +541,7 @@ __global__ __launch_bounds__(T, (T >= 512 ? 1 : carry_groups_per_cu<W, R, T>()))
     for (int c = 0; c < CPT; c++)
       if (W % T == 0 || tid + c * T < W) round_column(tid + c * T, c, fa_c[c], iwa_c[c]);
   } else {
-    for (int a = tid; a < W; a += T) round_column(a, 0, gload(A.fA + a), gload(A.iwA + a));
+    for (int a = tid; a < W; a += T) round_column(a, 0, gload(A.fA + a), gload(A.iwAs + a));
   }
 
   // carry-in from the previous block (block 0: the last block, one column
@@ -575,7 +585,7 @@ __global__ __launch_bounds__(T, (T >= 512 ? 1 : carry_groups_per_cu<W, R, T>()))
         }
         double d = balance_f64((e ? dv.y : dv.x) + cc, width, cc);
         if (i == 2 * R - 1) d += ldexp(cc, width);
-        x[e] = ldexp(d * (wa * wb[i]), -wrap);
+        x[e] = ldexp(mul3(d, wa, wb[i]), -wrap);
       }
       lds[r * LP + lpad(a)] = {x[0], x[1]};
     }
