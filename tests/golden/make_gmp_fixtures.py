@@ -98,6 +98,8 @@ def main():
         have = {e["iterations"] for e in data.get(str(p), [])}
         if not all(m in have for m in marks):
             todo.append((p, marks, quick))
+    # cheapest first (GMP's cost grows a little faster than p * iterations)
+    todo.sort(key=lambda j: j[0] * max(j[1]))
 
     def save(p, rows):
         old = {e["iterations"]: e for e in data.get(str(p), [])}
