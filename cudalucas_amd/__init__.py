@@ -28,6 +28,9 @@ LL_OPT_FORCE_SWEEPS = 1
 LL_OPT_NO_GRAPH = 2
 LL_OPT_GENERIC = 4
 
+# ll_info.path: LDS-resident, two sweeps, two sweeps on the generic kernels
+_PATHS = {0: "resident", 1: "sweeps", 2: "generic"}
+
 _lib = None
 
 
@@ -269,7 +272,7 @@ def fft_schedule(n, width=0, height=0, lines=0, flags=0):
     info = _Info()
     if lib().ll_fft_schedule(n, ctypes.byref(o), ctypes.byref(info)):
         return None
-    return ("resident" if info.path == 0 else "sweeps", info.width, info.height, info.lines)
+    return (_PATHS.get(info.path, "sweeps"), info.width, info.height, info.lines)
 
 
 def fft_from_str(s):
@@ -369,7 +372,7 @@ class LLContext:
         i = _Info()
         self._check(lib().ll_get_info(self._h, ctypes.byref(i)))
         return {
-            "p": i.p, "n": i.n, "path": "resident" if i.path == 0 else "sweeps",
+            "p": i.p, "n": i.n, "path": _PATHS.get(i.path, "sweeps"),
             "width": i.width, "height": i.height, "lines": i.lines,
             "stream": i.stream, "iterations": i.iterations,
         }

@@ -67,6 +67,17 @@ static int next_length(uint32_t q, int &idx, int n_failed) {
   return ll_choose_fft_length(q, &idx);
 }
 
+// The length to try after a roundoff failure at n: the next one larger than
+// n, or 0 if there is none (the policy's upper bound, CUDALucas.cu:3047,
+// where the list index stops growing).  idx moves only when there is one.
+static int larger_length(uint32_t q, int &idx, int n) {
+  int i = idx;
+  const int next = next_length(q, i, n);
+  if (next <= n) return 0;
+  idx = i;
+  return next;
+}
+
 static volatile sig_atomic_t g_quit = 0;
 
 static void on_signal(int sig) {
@@ -345,8 +356,6 @@ static int check(uint32_t q, const Config &cfg, const char *aid) {
     return 2;
   }
   const int base_idx = idx, base_n = n;
-  int32_t ub_lb[2];
-  ll_fft_bounds(q, &ub_lb[0], &ub_lb[1]);
   Engine eng;
   if (eng.open(cfg, q, (uint32_t)n)) return 2;
   std::vector<int32_t> digits(n);
@@ -377,8 +386,12 @@ static int check(uint32_t q, const Config &cfg, const char *aid) {
       const int r = round_off_test(eng, last < 1000 ? last : 1000);
       if (r < 0) return 2;
       if (r == 0) break;
-      n = next_length(q, idx, n);
-      if (n <= 0 || eng.open(cfg, q, (uint32_t)n) || load_snapshot(eng, good)) return 2;
+      n = larger_length(q, idx, n);
+      if (n <= 0) {
+        printf("No larger FFT length for M%u. Quitting.\n\n", q);
+        return 2;
+      }
+      if (eng.open(cfg, q, (uint32_t)n) || load_snapshot(eng, good)) return 2;
     }
     // the test ran real iterations: continue from where they left off
     digits.assign(n, 0);
@@ -432,9 +445,12 @@ static int check(uint32_t q, const Config &cfg, const char *aid) {
           break;
         }
       printf("Round off error at iteration = %u, err = %0.5g > %0.5g, fft = %dK.\n", at, m, error_limit, n / 1024);
-      if (n >= ub_lb[1] && idx + 1 >= ll_fft_length_count()) {
-        printf("Something is wrong! Quitting.\n\n");
-        return 2;
+      {
+        int probe = idx;
+        if (larger_length(q, probe, n) <= 0) {  // at the upper bound (CUDALucas.cu:3047-3051)
+          printf("Something is wrong! Quitting.\n\n");
+          return 2;
+        }
       }
       if (retry) {
         printf("Restarting from last checkpoint to see if the error is repeatable.\n\n");
@@ -444,7 +460,7 @@ static int check(uint32_t q, const Config &cfg, const char *aid) {
         return 2;
       } else {
         printf("The error persists.\nTrying a larger fft until the next checkpoint.\n\n");
-        n = next_length(q, idx, n);
+        n = larger_length(q, idx, n);
         if (n <= 0) return 2;
         fft_reset = true;
       }
@@ -617,6 +633,7 @@ static int check_residue(const Config &cfg, int longtest) {
     int n = ll_choose_fft_length(q, &idx);
     float m = 0.f;
     uint64_t res = 0;
+    bool failed = false;
     for (;;) {
       if (n <= 0) {
         printf("No FFT length available for M%u, skipped\n", q);
@@ -633,9 +650,14 @@ static int check_residue(const Config &cfg, int longtest) {
         fprintf(stderr, "%s\n", ll_last_error(e.ctx));
         return 2;
       }
-      if (m > 0.4f) {  // CUDALucas.cu:2767-2771
-        idx++;
-        n = ll_choose_fft_length(q, &idx);
+      if (m > 0.4f) {  // CUDALucas.cu:2767-2771, bounded by the list's upper bound
+        const int bigger = larger_length(q, idx, n);
+        if (bigger <= 0) {
+          printf("Round off error %5.5f > 0.4 at the largest allowed fft length %dK for M%u\n", m, n / 1024, q);
+          failed = true;
+          break;
+        }
+        n = bigger;
         continue;
       }
       uint32_t offset = 0;
@@ -649,6 +671,10 @@ static int check_residue(const Config &cfg, int longtest) {
       break;
     }
     if (n <= 0) continue;
+    if (failed) {
+      bad++;
+      continue;
+    }
     if (res != tab[i].res64) {
       printf("Expected residue [%016llx] does not match actual residue [%016llx]\n\n",
              (unsigned long long)tab[i].res64, (unsigned long long)res);

@@ -808,7 +808,7 @@ int ll_get_info(const ll_ctx *c, ll_info *info) {
   if (!c || !info) return LL_ERR_ARG;
   info->p = c->p;
   info->n = c->n;
-  info->path = c->sch.path;
+  info->path = c->sch.generic ? 2 : c->sch.path;
   info->width = c->sch.path ? c->sch.W : c->m;
   info->height = c->sch.path ? c->sch.H : 1;
   info->lines = c->sch.path ? c->sch.R : 1;
@@ -842,6 +842,10 @@ namespace {
 
 constexpr uint32_t kBatchRing = 1024;  // iterations per batch step at most
 constexpr uint32_t kBatchLds = 40 * 1024;  // LDS per batched workgroup: 4 per CU
+// Two-sweep tests up to this length run in the single-launch iteration on
+// the generic bodies; longer ones, where the specialised kernels are
+// 1.7x faster, keep a square and a carry launch per schedule group.
+constexpr uint32_t kBatchFusedMaxN = 262144;
 
 struct BatchGroup {
   Schedule sch{};
@@ -924,10 +928,19 @@ static int balloc(ll_batch *b, T **ptr, size_t count) {
 // one launch per iteration (batch_iteration) whose workgroups are every
 // test's square items, then every test's carry items.
 static int enqueue_sweeps(ll_batch *b, uint32_t iters) {
-  if (!b->item_count) return LL_OK;
-  for (uint32_t i = 0; i < iters; i++)
-    BCHK(b, launch_batch_iteration(b->d_items, b->item_count, b->d_fargs, b->d_sq_done, b->d_ca_done, b->lds,
-                                   b->stream));
+  for (uint32_t i = 0; i < iters; i++) {
+    if (b->item_count)
+      BCHK(b, launch_batch_iteration(b->d_items, b->item_count, b->d_fargs, b->d_sq_done, b->d_ca_done,
+                                     b->lds, b->stream));
+    // the longer tests: one square and one carry launch per schedule group,
+    // on the same stream, so no two carry launches run side by side
+    for (auto &grp : b->groups)
+      if (grp.sch.path == 1 && !grp.sch.generic) {
+        const uint32_t k = (uint32_t)grp.members.size();
+        BCHK(b, launch_square(grp.sch.sk, grp.d_args, k, grp.sch.W, grp.sch.H, b->stream));
+        BCHK(b, launch_carry(grp.sch.ck, grp.d_args, k, grp.sch.W, grp.sch.H, grp.sch.R, b->stream));
+      }
+  }
   return LL_OK;
 }
 
@@ -1015,6 +1028,7 @@ int ll_batch_create(int32_t device, uint32_t count, const ll_batch_test *tests, 
     memset(&gpick, 0, sizeof gpick);
     ll_options oi = o;
     if (pick_schedule(tests[i].n, &o, &pick) && pick.path == 1 &&
+        (pick.generic || tests[i].n <= kBatchFusedMaxN) &&
         pick_generic(tests[i].n / 2, 0, nullptr, &gpick, kBatchLds)) {
       oi.flags |= LL_OPT_GENERIC;
       oi.width = gpick.W;
@@ -1071,7 +1085,7 @@ int ll_batch_create(int32_t device, uint32_t count, const ll_batch_test *tests, 
   size_t items = 0;
   for (uint32_t i = 0; i < count; i++) {
     const ll_ctx *c = b->tests[i];
-    if (c->sch.path != 1) continue;
+    if (c->sch.path != 1 || !c->sch.generic) continue;
     PlanArgs a = c->args;
     a.remaining = b->d_remaining + i;
     a.err = b->d_ring + (size_t)i * kBatchRing;
@@ -1121,12 +1135,15 @@ int ll_batch_step(ll_batch *b, uint32_t iterations, float *step_max_err) {
                            hipMemcpyHostToDevice, b->stream2));
     BCHK(b, launch_resident(g.sch.rk, g.d_desc, (uint32_t)g.desc.size(), b->stream2));
   }
-  // two-sweep tests: one launch per iteration, graph-replayed
+  // two-sweep tests: one launch per iteration (and the longer tests' group
+  // launches), graph-replayed
   {
     int rc = build_items(b);
     if (rc) return rc;
   }
-  if (b->item_count) {
+  bool any_sweeps = b->item_count > 0;
+  for (auto &g : b->groups) any_sweeps |= g.sch.path == 1 && !g.sch.generic;
+  if (any_sweeps) {
     uint32_t left = iterations;
     if (left >= kGraphIters) {
       int rc = batch_graph(b);
@@ -1302,7 +1319,7 @@ int ll_fft_schedule(uint32_t n, const ll_options *opts, ll_info *info) {
   if (!pick_schedule(n, opts, &s)) return LL_ERR_UNSUPPORTED;
   memset(info, 0, sizeof(*info));
   info->n = n;
-  info->path = s.path;
+  info->path = s.generic ? 2 : s.path;
   if (s.path == 0) {
     info->width = n / 2;
     info->height = 1;

@@ -27,13 +27,26 @@ constexpr double kDomainHigh = 1.25;     // ... (training also covers up to 1.25
 constexpr double kMaxBitsPerWord = 28.0; // and never past 28 bits per word
 constexpr double kLongerLimit = 2.0;     // timed choices may go up to 2 n_ref
 
+// The selector's lengths: multiples of 1K words with a specialised
+// schedule (the lengths its roundoff traces cover), to which each exponent's
+// reference-policy length is added even where only the generic kernels run
+// it.
 const std::vector<uint32_t> &supported_lengths() {
   static std::vector<uint32_t> v = [] {
     std::vector<uint32_t> out;
-    for (uint32_t k = 1; k <= kMaxLengthK; k++)
-      if (ll_fft_supported(k * 1024u)) out.push_back(k * 1024u);
+    for (uint32_t k = 1; k <= kMaxLengthK; k++) {
+      ll_info info;
+      if (ll_fft_schedule(k * 1024u, nullptr, &info) == LL_OK && info.path != 2) out.push_back(k * 1024u);
+    }
     return out;
   }();
+  return v;
+}
+
+// lengths plus nref, ascending
+std::vector<uint32_t> with_reference(const std::vector<uint32_t> &lengths, uint32_t nref) {
+  std::vector<uint32_t> v = lengths;
+  if (!std::binary_search(v.begin(), v.end(), nref)) v.insert(std::lower_bound(v.begin(), v.end(), nref), nref);
   return v;
 }
 
@@ -91,7 +104,7 @@ int ll_selector_candidates(uint32_t p, uint32_t *ns, uint32_t capacity) {
   if (nref <= 0) return LL_ERR_UNSUPPORTED;
   const double lo = std::max(kDomainLow * nref, p / kMaxBitsPerWord), hi = kDomainHigh * nref;
   uint32_t k = 0;
-  for (uint32_t n : supported_lengths()) {
+  for (uint32_t n : with_reference(supported_lengths(), (uint32_t)nref)) {
     if (n < lo || n > hi) continue;
     if (ns) {
       if (k >= capacity) return LL_ERR_ARG;
@@ -222,7 +235,7 @@ int ll_selector_choose(ll_selector *s, uint32_t p, uint32_t above, float *pred_e
   // no longer than the reference's.
   const bool timed = !s->times.empty();
   std::vector<uint32_t> cand, longer;
-  for (uint32_t n : supported_lengths()) {
+  for (uint32_t n : with_reference(supported_lengths(), (uint32_t)nref)) {
     if (n < kDomainLow * nref || n < p / kMaxBitsPerWord || n <= above) continue;
     if (!above && n > (timed ? kLongerLimit * nref : (double)nref)) break;
     if (n > kDomainHigh * nref && !above) longer.push_back(n);

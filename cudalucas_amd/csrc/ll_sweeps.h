@@ -399,9 +399,14 @@ __global__ __launch_bounds__(256) void line_pass_table(const double *qc, cplx *o
   build_wave_table<W, 256, auto_plan_uniform<W, R, T>()>(out, QuarterRoots<W>{qc});
 }
 
-// Workgroups of carry_sweep<W, R, T> that the LDS of a CU holds, at most 2:
-// the launch bound asks for no more occupancy than that, so a sweep whose
-// lines fill over half the LDS is not held to half the register file.
+// Workgroups of carry_sweep<W, R, T> that the LDS of a CU holds, at most 2.
+// HIP passes the second __launch_bounds__ argument on as the minimum waves
+// per SIMD (amdgpu_waves_per_eu), which bounds the registers per lane; with
+// T = 256 (one wave per SIMD per workgroup) the value is also workgroups per
+// CU, so a sweep whose lines fill over half the LDS is not held to half the
+// register file.  For T = 64 and 128 the value asks for more waves per SIMD
+// than one workgroup brings (a register bound only; the LDS still limits
+// how many workgroups share a CU).
 template <int W, int R, int T>
 constexpr int carry_groups_per_cu() {
   constexpr bool tab = carry_pass_table<W, R, T>();

@@ -306,8 +306,26 @@ def read_times(path):
 
 
 def supported_lengths(max_k=16384):
-    L = lib()
-    return [k * 1024 for k in range(1, max_k + 1) if L.ll_fft_supported(k * 1024)]
+    """Multiples of 1K words with a specialised schedule (the selector's
+    candidates; lengths only the generic kernels run are left out)."""
+    from . import fft_schedule
+
+    out = []
+    for k in range(1, max_k + 1):
+        s = fft_schedule(k * 1024)
+        if s is not None and s[0] != "generic":
+            out.append(k * 1024)
+    return out
+
+
+def timed_lengths(max_k=16384):
+    """Lengths the timing table covers: the selector's candidates and every
+    length of the reference's default list up to max_k K words (an
+    exponent's reference-policy length may be one only the generic kernels
+    run)."""
+    from . import fft_lengths
+
+    return sorted(set(supported_lengths(max_k)) | {n for n in fft_lengths() if n <= max_k * 1024})
 
 
 # -------------------------------------------------------------- selector
@@ -611,7 +629,7 @@ def main(argv=None):
             with open(args.stats, "w") as f:
                 json.dump(stats, f, indent=1)
     elif args.cmd == "times":
-        save_times(bench_lengths(supported_lengths(), args.iters, log=_log), args.out)
+        save_times(bench_lengths(timed_lengths(), args.iters, log=_log), args.out)
     elif args.cmd == "evaluate":
         times = read_times(args.times)
         primes = W.log_uniform_primes(W.SELECTOR_EVAL_SEED, args.count, *W.SELECTOR_EVAL_RANGE)

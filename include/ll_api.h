@@ -58,7 +58,8 @@ typedef struct ll_options {
 
 typedef struct ll_info {
   uint32_t p, n;
-  uint32_t path; /* 0 = LDS-resident single workgroup, 1 = two HBM sweeps */
+  uint32_t path; /* 0 = LDS-resident single workgroup, 1 = two HBM sweeps,
+                    2 = two HBM sweeps on the generic (runtime-plan) kernels */
   uint32_t width, height, lines;
   void *stream;  /* the hipStream_t all work of this context runs on */
   uint64_t iterations; /* iterations run since the last load */

@@ -159,9 +159,11 @@ def test_fft_length_policy_matches_oracle():
         assert n == first
     # manual override (an index beyond the list is a length in words) is
     # clamped to [lb, ub] like the reference: 4M is below lb(77M) = 4216441,
-    # so it is raised to the next list length this build has kernels for.
+    # so it is raised to the next list length, 4320K (every list length has
+    # kernels in this build)
     n, _ = L.choose_fft_length(77005277, 4194304)
-    assert n == 5242880
+    assert n == 4423680
+    assert all(L.fft_supported(x) for x in O.default_fft_lengths())
     n, _ = L.choose_fft_length(75846319, 4194304)
     assert n == 4194304
     n, _ = L.choose_fft_length(44497, 4096)

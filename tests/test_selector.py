@@ -157,5 +157,5 @@ def test_committed_model_is_well_formed():
 @pytest.mark.skipif(not os.path.exists(S.TIMES_PATH), reason="no timing table committed")
 def test_committed_times_cover_every_supported_length():
     t = S.read_times(S.TIMES_PATH)
-    assert set(S.supported_lengths()) <= set(t)
+    assert set(S.timed_lengths()) <= set(t)
     assert all(v > 0 for v in t.values())
