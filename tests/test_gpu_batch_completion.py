@@ -9,8 +9,8 @@ no iteration of any test exceeds the 0.35 roundoff limit (so there is no
 undetected failure to look for); and for the smallest exponents the full
 final residue equals the plain single-exponent engine's, run to completion
 beside it.  The whole run takes about half an hour of GPU time, so by default
-only the exponents below 1e6 run (with the Mersenne primes 110503 ...
-859433); the four parts of the whole run are asked for by name:
+only the exponents below 8e5 run (with the Mersenne primes 110503 ...
+756839); the four parts of the whole run are asked for by name:
 
     LL_BATCH_COMPLETION=low  python -m pytest -s tests/test_gpu_batch_completion.py   # p < 2e6
     LL_BATCH_COMPLETION=mid  python -m pytest -s tests/test_gpu_batch_completion.py   # 2e6 <= p < 3.5e6
@@ -33,7 +33,7 @@ from conftest import announce
 pytestmark = pytest.mark.gpu
 
 MERSENNE_PRIME_EXPONENTS = [110503, 132049, 216091, 756839, 859433, 1257787, 1398269, 2976221, 3021377]
-PARTS = {"default": (0, 1_000_000), "low": (0, 2_000_000), "mid": (2_000_000, 3_500_000),
+PARTS = {"default": (0, 800_000), "low": (0, 2_000_000), "mid": (2_000_000, 3_500_000),
          "high": (3_500_000, 4_300_000), "top": (4_300_000, 10_000_000)}
 PLAIN_CHECKS = 24  # smallest exponents also run to completion in the plain engine
 PART = os.environ.get("LL_BATCH_COMPLETION", "") or "default"

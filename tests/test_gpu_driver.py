@@ -123,9 +123,11 @@ def test_resume_from_reference_format_checkpoint(tmp_path):
 
 
 def test_roundoff_error_retry_then_larger_fft(tmp_path):
-    p = W.SMALL_P
+    # 8K is inside [lb, ub] = [5705, 14336] for this exponent, so 10K and
+    # then 14K may follow
+    p = 120223
     # once: retried from the last report state and the error goes away
-    r = drive(["-f", "4096", "-x", "5000", "--offset", "5", "--inject-error", "7001:0.45:1", p], tmp_path)
+    r = drive(["-f", "8192", "-x", "5000", "--offset", "5", "--inject-error", "7001:0.45:1", p], tmp_path)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "Round off error at iteration = 7001" in r.stdout
     assert "Restarting from last checkpoint to see if the error is repeatable." in r.stdout
@@ -133,14 +135,26 @@ def test_roundoff_error_retry_then_larger_fft(tmp_path):
     assert "M( %d )C, 0x%016x" % (p, gmp_res64(p, p - 2)) in result_line(tmp_path)
     # twice: a larger FFT until the next report, then back (CUDALucas.cu:3043-3088)
     (tmp_path / "results.txt").unlink()
-    r = drive(["-f", "4096", "-x", "5000", "--offset", "5", "--inject-error", "7001:0.45:2", p], tmp_path)
+    r = drive(["-f", "8192", "-x", "5000", "--offset", "5", "--inject-error", "7001:0.45:2", p], tmp_path)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "The error persists." in r.stdout and "Trying a larger fft until the next checkpoint." in r.stdout
     assert "Resetting fft." in r.stdout
     assert "M( %d )C, 0x%016x" % (p, gmp_res64(p, p - 2)) in result_line(tmp_path)
     # three times: a larger FFT did not help either
-    r = drive(["-f", "4096", "-x", "5000", "--offset", "5", "--inject-error", "7001:0.45:3", p], tmp_path)
+    r = drive(["-f", "8192", "-x", "5000", "--offset", "5", "--inject-error", "7001:0.45:3", p], tmp_path)
     assert r.returncode == 2 and "The error won't go away. I give up." in r.stdout
+
+
+def test_roundoff_error_at_the_upper_bound_quits(tmp_path):
+    """At the largest length the policy allows (ub(42839) = 5491 words, so
+    no list length above 4K), a roundoff error ends the run
+    (CUDALucas.cu:3047-3051) instead of retrying the same length."""
+    p = W.SMALL_P
+    r = drive(["-f", "4096", "-x", "5000", "--offset", "5", "--inject-error", "7001:0.45:2", p], tmp_path)
+    assert r.returncode == 2, r.stdout + r.stderr
+    assert "Round off error at iteration = 7001" in r.stdout
+    assert "Something is wrong! Quitting." in r.stdout
+    assert "Trying a larger fft" not in r.stdout
 
 
 def test_selftest_short_table(tmp_path):
