@@ -1,13 +1,20 @@
 // Engine contexts behind the C-ABI (include/ll_api.h): schedule choice,
 // device tables and buffers, the iteration loop with hipGraph replay,
 // load/unload of digits, per-iteration error readback.
+#include <errno.h>
+#include <fcntl.h>
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
+#include <sys/file.h>
+#include <unistd.h>
 
 #include <algorithm>
+#include <memory>
+#include <mutex>
 #include <vector>
 
 #include "ll_api.h"
@@ -17,6 +24,59 @@
 using namespace llamd;
 
 namespace {
+
+// A carry sweep waits on other workgroups of its own launch, which each XCD
+// dispatches in order; two such launches side by side on one GPU can hold
+// the workgroup slots each other's awaited workgroups need.  So every call
+// that runs carry sweeps holds its GPU's lock from its first launch to its
+// synchronisation: a mutex for the threads of this process, and an
+// exclusive flock on "<tmp>/cudalucas_amd-gpu-<PCI bus id>.lock" (TMPDIR,
+// else /tmp) for other processes using this library on the same GPU.
+// Where the lock file cannot be opened only the in-process mutex applies.
+class GpuLock {
+ public:
+  explicit GpuLock(int device) : dev_(device >= 0 && device < kDevices ? device : 0) {
+    mutex_[dev_].lock();
+    const int fd = lock_fd(dev_);
+    if (fd >= 0) {
+      while (flock(fd, LOCK_EX) != 0 && errno == EINTR) {
+      }
+      locked_ = true;
+    }
+  }
+  ~GpuLock() {
+    if (locked_) (void)flock(fds_[dev_], LOCK_UN);
+    mutex_[dev_].unlock();
+  }
+  GpuLock(const GpuLock &) = delete;
+  GpuLock &operator=(const GpuLock &) = delete;
+
+ private:
+  static constexpr int kDevices = 64;
+  static std::mutex mutex_[kDevices];
+  static int fds_[kDevices];
+  // Opened once per device (under the device's mutex) and kept open.
+  static int lock_fd(int dev) {
+    if (fds_[dev] != -1) return fds_[dev];
+    char bus[64] = {0};
+    if (hipDeviceGetPCIBusId(bus, sizeof bus, dev) != hipSuccess) snprintf(bus, sizeof bus, "dev%d", dev);
+    for (char *q = bus; *q; q++)
+      if (*q == ':' || *q == '.' || *q == '/') *q = '_';
+    const char *tmp = getenv("TMPDIR");
+    char path[512];
+    snprintf(path, sizeof path, "%s/cudalucas_amd-gpu-%s.lock", tmp && *tmp ? tmp : "/tmp", bus);
+    const int fd = open(path, O_RDWR | O_CREAT | O_CLOEXEC, 0666);
+    fds_[dev] = fd >= 0 ? fd : -2;
+    return fds_[dev];
+  }
+  int dev_;
+  bool locked_ = false;
+};
+std::mutex GpuLock::mutex_[GpuLock::kDevices];
+int GpuLock::fds_[GpuLock::kDevices] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1,
+                                        -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1,
+                                        -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1,
+                                        -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
 
 static_assert(LL_NOALLOC_SCRATCH_BYTES == kNoAllocScratchBytes, "the header states the scratch size");
 
@@ -270,7 +330,8 @@ template <typename T>
 static int upload(ll_ctx *c, T **ptr, const std::vector<T> &v) {
   int rc = dalloc(c, ptr, v.size());
   if (rc) return rc;
-  HIPCHK(c, hipMemcpy(*ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpyAsync(*ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return LL_OK;
 }
 
@@ -372,9 +433,9 @@ static int build_sweeps(ll_ctx *c) {
   const size_t shuttle = (size_t)(H / R) * W;
   c->shuttle_bytes = shuttle * sizeof(uint64_t);
   if ((rc = dalloc(c, &A.shuttle, shuttle))) return rc;
-  HIPCHK(c, hipMemset(A.shuttle, 0xff, shuttle * sizeof(uint64_t)));
+  HIPCHK(c, hipMemsetAsync(A.shuttle, 0xff, shuttle * sizeof(uint64_t), c->stream));
   if ((rc = dalloc(c, &c->st, 1))) return rc;
-  HIPCHK(c, hipMemset(c->st, 0, sizeof(DevState)));
+  HIPCHK(c, hipMemsetAsync(c->st, 0, sizeof(DevState), c->stream));
   if ((rc = dalloc(c, &c->ring, c->ring_size))) return rc;
   A.x = c->x;
   A.st = c->st;
@@ -393,17 +454,19 @@ static int build_sweeps(ll_ctx *c) {
   A.remaining = nullptr;
   if ((rc = dalloc(c, &c->d_args, 1))) return rc;
   if ((rc = dalloc(c, &c->d_args_unload, 1))) return rc;
-  HIPCHK(c, hipMemcpy(c->d_args, &A, sizeof(PlanArgs), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpyAsync(c->d_args, &A, sizeof(PlanArgs), hipMemcpyHostToDevice, c->stream));
   PlanArgs U = A;
   U.err = c->unload_err;
-  HIPCHK(c, hipMemcpy(c->d_args_unload, &U, sizeof(PlanArgs), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpyAsync(c->d_args_unload, &U, sizeof(PlanArgs), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   if (c->shuttle_bytes <= kNoAllocShuttleBytes) {
     PlanArgs S = A;
     S.shuttle = (uint64_t *)c->scratch;
     S.err = (uint32_t *)(c->scratch + kNoAllocShuttleBytes);
     S.err_mask = kNoAllocRing - 1;
     if ((rc = dalloc(c, &c->d_args_static, 1))) return rc;
-    HIPCHK(c, hipMemcpy(c->d_args_static, &S, sizeof(PlanArgs), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpyAsync(c->d_args_static, &S, sizeof(PlanArgs), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
   }
   return LL_OK;
 }
@@ -450,7 +513,8 @@ static void release(ll_ctx *c) {
 static int build_graph(ll_ctx *c) {
   if (c->graph) return LL_OK;
   hipGraph_t g = nullptr;
-  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  // relaxed: other threads may create contexts (and copy) while this captures
+  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
   for (uint32_t i = 0; i < kGraphIters; i++) {
     (void)launch_square(c->sch.sk, c->d_args, 1, c->sch.W, c->sch.H, c->stream);
     (void)launch_carry(c->sch.ck, c->d_args, 1, c->sch.W, c->sch.H, c->sch.R, c->stream);
@@ -643,17 +707,17 @@ static int iterate_common(ll_ctx *c, uint32_t count, float *per_iter, float *max
   uint32_t slots = c->ring_size;
   if (noalloc) {
     slots = std::min(slots, kNoAllocRing);
-    if (c->sch.path == 1) {
-      if (!c->d_args_static)
-        return fail(c, LL_ERR_ARG, "carry shuttle of %zu bytes exceeds the %zu-byte no-allocation scratch",
-                    c->shuttle_bytes, kNoAllocShuttleBytes);
-      // stale carries of other contexts must not carry this epoch's tag
-      HIPCHK(c, launch_fill((uint32_t *)c->scratch, 0xffffffffu, c->shuttle_bytes / 4, c->stream));
-    }
+    if (c->sch.path == 1 && !c->d_args_static)
+      return fail(c, LL_ERR_ARG, "carry shuttle of %zu bytes exceeds the %zu-byte no-allocation scratch",
+                  c->shuttle_bytes, kNoAllocShuttleBytes);
   }
   c->last_gpu_ms = 0.0;
   float m = 0.f;
   uint32_t done = 0;
+  std::unique_ptr<GpuLock> lock;
+  if (c->sch.path == 1) lock.reset(new GpuLock(c->device));
+  if (noalloc && c->sch.path == 1)  // stale carries of other contexts must not carry this epoch's tag
+    HIPCHK(c, launch_fill((uint32_t *)c->scratch, 0xffffffffu, c->shuttle_bytes / 4, c->stream));
   while (done < count) {
     const uint32_t chunk = std::min<uint32_t>(count - done, slots);
     int rc;
@@ -743,6 +807,7 @@ int ll_profile_kernels(ll_ctx *c, uint32_t count, float *kernel_ms) {
     return LL_OK;
   }
   double sq = 0.0, ca = 0.0;
+  GpuLock lock(c->device);
   for (uint32_t i = 0; i < count; i++) {
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
     HIPCHK(c, launch_square(c->sch.sk, c->d_args, 1, c->sch.W, c->sch.H, c->stream));
@@ -757,7 +822,10 @@ int ll_profile_kernels(ll_ctx *c, uint32_t count, float *kernel_ms) {
     ca += b;
   }
   uint32_t off = 0;
-  HIPCHK(c, hipMemcpy(&off, &c->st->offset, sizeof(uint32_t), hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpyAsync(&c->h_words->offset, &c->st->offset, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  off = c->h_words->offset;
   c->offset = off;
   c->epoch += count;
   c->iterations += count;
@@ -970,14 +1038,16 @@ static int build_items(ll_batch *b) {
   b->in_items = in;
   b->item_count = (uint32_t)items.size();
   if (!items.empty())
-    BCHK(b, hipMemcpy(b->d_items, items.data(), items.size() * sizeof(BatchItem), hipMemcpyHostToDevice));
+    BCHK(b, hipMemcpyAsync(b->d_items, items.data(), items.size() * sizeof(BatchItem), hipMemcpyHostToDevice,
+                           b->stream));
+  BCHK(b, hipStreamSynchronize(b->stream));
   return LL_OK;
 }
 
 static int batch_graph(ll_batch *b) {
   if (b->graph) return LL_OK;
   hipGraph_t g = nullptr;
-  BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
   const int rc = enqueue_sweeps(b, kGraphIters);
   const hipError_t ec = hipStreamEndCapture(b->stream, &g);
   if (rc) {
@@ -1056,7 +1126,7 @@ int ll_batch_create(int32_t device, uint32_t count, const ll_batch_test *tests, 
     }
     grp->members.push_back(i);
   }
-  BCHK(b, hipMemcpy(b->d_remaining, b->remaining.data(), 4 * count, hipMemcpyHostToDevice));
+  BCHK(b, hipMemcpyAsync(b->d_remaining, b->remaining.data(), 4 * count, hipMemcpyHostToDevice, b->stream));
   for (auto &g : b->groups) {
     const size_t k = g.members.size();
     if (g.sch.path == 1) {
@@ -1068,7 +1138,8 @@ int ll_batch_create(int32_t device, uint32_t count, const ll_batch_test *tests, 
         a[t].err_mask = kBatchRing - 1;
       }
       if ((rc = balloc(b, &g.d_args, k))) return rc;
-      BCHK(b, hipMemcpy(g.d_args, a.data(), k * sizeof(PlanArgs), hipMemcpyHostToDevice));
+      BCHK(b, hipMemcpyAsync(g.d_args, a.data(), k * sizeof(PlanArgs), hipMemcpyHostToDevice, b->stream));
+      BCHK(b, hipStreamSynchronize(b->stream));
     } else {
       g.desc.resize(k);
       for (size_t t = 0; t < k; t++) {
@@ -1099,9 +1170,10 @@ int ll_batch_create(int32_t device, uint32_t count, const ll_batch_test *tests, 
     if ((rc = balloc(b, &b->d_fargs, fa.size())) || (rc = balloc(b, &b->d_sq_done, fa.size())) ||
         (rc = balloc(b, &b->d_ca_done, fa.size())) || (rc = balloc(b, &b->d_items, items)))
       return rc;
-    BCHK(b, hipMemcpy(b->d_fargs, fa.data(), fa.size() * sizeof(PlanArgs), hipMemcpyHostToDevice));
-    BCHK(b, hipMemset(b->d_sq_done, 0, fa.size() * sizeof(unsigned long long)));
-    BCHK(b, hipMemset(b->d_ca_done, 0, fa.size() * sizeof(unsigned long long)));
+    BCHK(b, hipMemcpyAsync(b->d_fargs, fa.data(), fa.size() * sizeof(PlanArgs), hipMemcpyHostToDevice, b->stream));
+    BCHK(b, hipMemsetAsync(b->d_sq_done, 0, fa.size() * sizeof(unsigned long long), b->stream));
+    BCHK(b, hipMemsetAsync(b->d_ca_done, 0, fa.size() * sizeof(unsigned long long), b->stream));
+    BCHK(b, hipStreamSynchronize(b->stream));
   }
   if ((rc = build_items(b))) return rc;
   return LL_OK;
@@ -1123,6 +1195,7 @@ int ll_batch_step(ll_batch *b, uint32_t iterations, float *step_max_err) {
   BCHK(b, hipSetDevice(b->device));
   const uint32_t count = (uint32_t)b->tests.size();
   std::vector<uint32_t> before = b->remaining;
+  GpuLock lock(b->device);
   // (error slots need no clearing: the resident kernel writes each of its
   // iterations' slots, and the square sweep clears its iteration's slot)
   BCHK(b, hipEventRecord(b->ev[0], b->stream));

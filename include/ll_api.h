@@ -15,10 +15,16 @@
  *
  * Concurrency: a context (or batch) runs its kernels on its own stream(s).
  * The carry sweep of the two-sweep path waits on other workgroups of the
- * same launch; two such launches sharing one GPU can hold each other's
- * workgroup slots, so iterate at most one two-sweep context or batch per
- * GPU at a time (a batch orders its own carry sweeps itself).  A wait that
- * does not end is reported as a failed iteration (error 3.0), never a hang.
+ * same launch; two such launches side by side on one GPU could hold each
+ * other's workgroup slots.  Every call that runs carry sweeps (ll_iterate,
+ * ll_iterate_noalloc, ll_profile_kernels, ll_batch_step) therefore holds the
+ * GPU's lock from its first launch to its synchronisation: a mutex for the
+ * threads of the process and an flock on
+ * "$TMPDIR/cudalucas_amd-gpu-<PCI bus id>.lock" (/tmp without TMPDIR) for
+ * other processes, so contexts in different threads or processes can
+ * iterate at the same time and their calls interleave.  One context must
+ * not be used from two threads at once.  A wait that does not end is
+ * reported as a failed iteration (error 3.0), never a hang.
  */
 #ifndef LL_API_H
 #define LL_API_H
