@@ -91,12 +91,12 @@ def cpu_baseline(p, n, budget_s):
     return out
 
 
-def batched_leg(steps=4, step_iters=512, plain_iters=200):
+def batched_leg(steps=4, step_iters=512, plain_iters=200, plain=True):
     """Batched workload: 256 seeded exponents log-uniform in [1e5, 5e6] at their
     chosen FFT lengths.  Exponents/hour, with finished tests replaced by new
     ones of the same mix: batched = 3600 * 256 / (t_step * mean(p - 2));
     plain (one exponent at a time on the same GPU) =
-    3600 / mean((p - 2) * t_iter(p))."""
+    3600 / mean((p - 2) * t_iter(p)), left out when `plain` is false."""
     import numpy as np
 
     import cudalucas_amd as L
@@ -124,30 +124,33 @@ def batched_leg(steps=4, step_iters=512, plain_iters=200):
     t_step = total_ms * 1e-3 / (steps * step_iters)
     mean_iters = float(np.mean([p - 2 for p in ps]))
     batched = 3600.0 * len(ps) / (t_step * mean_iters)
+    # algorithmic bytes of one batch iteration: 32 N per test (two read+write
+    # sweeps of its N-word FP64 signal; LDS-resident tests counted alike)
+    b_alg = 32.0 * float(sum(lens))
+    out = {
+        "workload": "256 seeded primes log-uniform in [1e5, 5e6], FFT lengths %dK-%dK (%d schedule groups)"
+        % (min(lens) // 1024, max(lens) // 1024, groups),
+        "exps_per_hour_batched": round(batched, 1),
+        "batch_iteration_ms": round(t_step * 1e3, 4),
+        "hbm_gbs_alg": round(b_alg / t_step / 1e9, 1),
+        "sample": "%d timed batch iterations after 64 warm-up" % (steps * step_iters),
+        "schedules": "batched autotuner table" if tuned else "built-in preference",
+        "max_roundoff": round(worst, 4),
+    }
+    if not plain:
+        return out
     per = []
     for p, n in zip(ps, lens):
         with L.LLContext(p, n) as c:
             c.load_initial(p % 1000)
-            c.iterate(16)
+            c.iterate(32)  # builds and replays the context's graph once
             c.iterate(plain_iters)
             per.append((p - 2) * c.last_iterate_ms() * 1e-3 / plain_iters)
-    plain = 3600.0 / float(np.mean(per))
-    # algorithmic bytes of one batch iteration: 32 N per test (two read+write
-    # sweeps of its N-word FP64 signal; LDS-resident tests counted alike)
-    b_alg = 32.0 * float(sum(lens))
-    return {
-        "workload": "256 seeded primes log-uniform in [1e5, 5e6], FFT lengths %dK-%dK (%d schedule groups)"
-        % (min(lens) // 1024, max(lens) // 1024, groups),
-        "exps_per_hour_batched": round(batched, 1),
-        "exps_per_hour_plain": round(plain, 1),
-        "speedup": round(batched / plain, 2),
-        "batch_iteration_ms": round(t_step * 1e3, 4),
-        "hbm_gbs_alg": round(b_alg / t_step / 1e9, 1),
-        "sample": "%d timed batch iterations after 64 warm-up; plain: %d iterations per exponent, all 256"
-        % (steps * step_iters, plain_iters),
-        "schedules": "batched autotuner table" if tuned else "built-in preference",
-        "max_roundoff": round(worst, 4),
-    }
+    plain_rate = 3600.0 / float(np.mean(per))
+    out["exps_per_hour_plain"] = round(plain_rate, 1)
+    out["speedup"] = round(batched / plain_rate, 2)
+    out["sample"] += "; plain: %d iterations per exponent after 32 warm-up, all 256" % plain_iters
+    return out
 
 
 def smooth_leg(iters=2000, warmup=100):
@@ -178,6 +181,33 @@ def smooth_leg(iters=2000, warmup=100):
             "hbm_frac_alg": round(b_alg / (gpu * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "max_roundoff": round(float(m), 4),
         })
+    return rows
+
+
+def lengths_leg(spec, iters=500, warmup=100):
+    """Extra lengths ("8192K,3136K,..."): the densest exponent the
+    reference's policy assigns to each, ms/iteration (hipEvent GPU time) and
+    HBM GB/s at 32 N bytes per iteration, with the schedule and kernel path
+    (specialised sweeps or the generic runtime-plan kernels)."""
+    import cudalucas_amd as L
+    from cudalucas_amd import workloads as W
+
+    rows = []
+    for tok in spec.split(","):
+        n = L.fft_from_str(tok.strip())
+        p = W.densest_prime(n)
+        with L.LLContext(p, n) as c:
+            info = c.info()
+            c.load_initial(0)
+            c.iterate(warmup)
+            m = c.iterate(iters)
+            gpu = c.last_iterate_ms() / iters
+        b_alg, _ = algorithmic(n)
+        rows.append({"n": n, "p": p, "bits_per_word": round(p / n, 3), "path": info["path"],
+                     "schedule": "W=%d H=%d R=%d" % (info["width"], info["height"], info["lines"]),
+                     "gpu_ms_per_iter": round(gpu, 5), "hbm_gbs_alg": round(b_alg / (gpu * 1e-3) / 1e9, 1),
+                     "hbm_frac_alg": round(b_alg / (gpu * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "max_roundoff": round(float(m), 4)})
     return rows
 
 
@@ -221,7 +251,10 @@ def main():
     ap.add_argument("--copy-only", action="store_true",
                     help="only the 64 MiB streaming-copy calibration kernel")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="batched leg without the one-exponent-at-a-time comparison")
     ap.add_argument("--no-smooth", action="store_true", help="skip the smooth-length and N=4096 legs")
+    ap.add_argument("--lengths", default="", help="extra lengths to time, e.g. 8192K,3136K (densest exponents)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: measures one GPU; the LL path of one exponent runs on a single "
@@ -318,8 +351,16 @@ def main():
             if single_tuned:
                 T.install([])
         out["small"] = small_leg()
+    if args.lengths:
+        if single_tuned:
+            T.install(T.load(T.SINGLE_TABLE))
+        try:
+            out["lengths"] = lengths_leg(args.lengths)
+        finally:
+            if single_tuned:
+                T.install([])
     if not args.no_batched:
-        out["batched"] = batched_leg()
+        out["batched"] = batched_leg(plain=not args.no_plain)
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(p, n, args.cpu_budget)
