@@ -480,16 +480,25 @@ __device__ __forceinline__ void stockham_pass_wave(cplx *buf, const TW &tw) {
 }
 
 // NB transforms of length L, each wave transforming its own lines; callers
-// synchronise the workgroup before and after.
+// synchronise the workgroup before and after.  Passes PASS .. END - 1 are
+// run (END = -1: to the last pass).
 template <int L, int NB, int T, bool INV, bool UNI = wave_plan_ok<L, NB, T, true>(), int PASS = 0,
-          int NS = 1, typename TW>
+          int NS = 1, int END = -1, typename TW>
 __device__ __forceinline__ void fft_lds_wave(cplx *buf, const TW &tw) {
   static_assert(wave_plan_ok<L, NB, T, UNI>(), "fft_lds_wave: plan is not wave-local");
-  if constexpr (NS < L) {
+  if constexpr (NS < L && PASS != END) {
     constexpr int R = wave_radix<UNI>(L, PASS);
     stockham_pass_wave<L, R, NS, NB, T, INV>(buf, tw);
-    fft_lds_wave<L, NB, T, INV, UNI, PASS + 1, NS * R>(buf, tw);
+    fft_lds_wave<L, NB, T, INV, UNI, PASS + 1, NS * R, END>(buf, tw);
   }
+}
+
+// Number of passes of a wave-local plan.
+template <bool UNI>
+constexpr int wave_passes(int L) {
+  int n = 0, ns = 1;
+  while (ns < L) ns *= wave_radix<UNI>(L, n++);
+  return n;
 }
 
 // Either form, chosen at compile time.  With TAB the wave-local form takes

@@ -415,6 +415,23 @@ constexpr int carry_groups_per_cu() {
   return 163840 / bytes >= 2 ? 2 : 1;
 }
 
+// Shape of carry_sweep's fused end passes: the wave-local plan (UNI, NP
+// passes, first radix R0, last radix RL) and whether both end passes map
+// onto the threads' load and store positions (M0 first-pass butterflies
+// per thread).
+template <int W, int R, int T>
+struct CarryEnds {
+  static constexpr bool wl = wave_local_ok<W, R, T>();
+  static constexpr bool UNI = auto_plan_uniform<W, R, T>();
+  static constexpr int NP = wl ? wave_passes<UNI>(W) : 0;
+  static constexpr int R0 = wl ? wave_radix<UNI>(W, 0) : 1;
+  static constexpr int RL = wl ? wave_radix<UNI>(W, NP - 1) : 1;
+  static constexpr int TA = T / R;
+  static constexpr bool ok = wl && NP >= 2 && T % R == 0 && (R * W) % T == 0 && (W / R0) % TA == 0 &&
+                             (W / RL) % TA == 0;
+  static constexpr int M0 = ok ? W / R0 / TA : 1;
+};
+
 template <int W, int R, int T>
 __global__ __launch_bounds__(T, (T >= 512 ? 1 : carry_groups_per_cu<W, R, T>())) void carry_sweep(
     const PlanArgs *__restrict__ args) {
@@ -429,6 +446,15 @@ __global__ __launch_bounds__(T, (T >= 512 ? 1 : carry_groups_per_cu<W, R, T>()))
   constexpr bool TAB = carry_pass_table<W, R, T>();
   __shared__ cplx ptab[TAB ? W : 1];  // wave-local pass twiddles (copy of ptW)
   const int tid = threadIdx.x;
+  // Fused end passes (wave-local plans whose first and last passes map onto
+  // the threads' load and store positions): thread t holds elements
+  // e = t + s T, i.e. line r = t mod R and columns a = t / R + TA s
+  // (TA = T / R).  The inverse transform's first pass (radix R0, inputs
+  // a = j + Q0 i) then runs in registers straight from the loads, and the
+  // forward transform's last pass (radix RL, outputs a = j + QL i) goes
+  // straight to HBM, each saving an LDS round trip of the whole block.
+  using Fused = CarryEnds<W, R, T>;
+  constexpr bool FUSE = Fused::ok;
   const uint32_t H = A.H;
   const uint32_t G = H / R;
   const uint32_t blk = blockIdx.x;
@@ -474,17 +500,40 @@ __global__ __launch_bounds__(T, (T >= 512 ? 1 : carry_groups_per_cu<W, R, T>()))
       wb[tid] = gload(A.wB + i);
       iwb[tid] = gload(A.iwB + i);
     }
+    if constexpr (FUSE) {
+      // first inverse pass (span 1, no twiddles): butterfly j of line r
+      // takes columns j + Q0 i, which this thread loaded as v[s0 + M0 i]
+      constexpr int R0 = Fused::R0, M0 = Fused::M0, TA = T / R;
+      const int r = tid % R, j0 = tid / R;
 #pragma unroll
-    for (int s = 0; s < NE; s++) {
-      const int e = tid + s * T;
-      if ((R * W) % T == 0 || e < R * W) lds[(e % R) * LP + lpad(e / R)] = v[s];
+      for (int s0 = 0; s0 < M0; s0++) {
+        cplx x[R0];
+#pragma unroll
+        for (int i = 0; i < R0; i++) x[i] = v[s0 + M0 * i];
+        dft<R0, true>(x);
+        const int j = j0 + TA * s0;
+#pragma unroll
+        for (int i = 0; i < R0; i++) lds[r * LP + lpad(j * R0 + i)] = x[i];
+      }
+    } else {
+#pragma unroll
+      for (int s = 0; s < NE; s++) {
+        const int e = tid + s * T;
+        if ((R * W) % T == 0 || e < R * W) lds[(e % R) * LP + lpad(e / R)] = v[s];
+      }
     }
   }
   const uint32_t epoch = A.st->epoch;
   const uint32_t m2_digit = A.st->m2_digit;
   const double dmod = (double)A.st->m2_value;
   __syncthreads();
-  fft_lds_auto<W, R, T, true, TAB>(lds, tw, ptab);
+  if constexpr (FUSE) {
+    if constexpr (TAB) fft_lds_wave<W, R, T, true, Fused::UNI, 1, Fused::R0>(lds, WaveTable{ptab});
+    else fft_lds_wave<W, R, T, true, Fused::UNI, 1, Fused::R0>(lds, tw);
+    __syncthreads();
+  } else {
+    fft_lds_auto<W, R, T, true, TAB>(lds, tw, ptab);
+  }
 
   // Round, subtract, carry along the 2R consecutive words of each column,
   // in exact FP64 (balance_f64).  The balanced digits go back into the
@@ -603,8 +652,42 @@ __global__ __launch_bounds__(T, (T >= 512 ? 1 : carry_groups_per_cu<W, R, T>()))
     for (int a = tid; a < W; a += T) carry_in_column(a, 0, gload(A.fA + a), gload(A.wA + a));
   }
   __syncthreads();
-  fft_lds_auto<W, R, T, false, TAB>(lds, tw, ptab);
-  {
+  if constexpr (FUSE) {
+    constexpr int NPW = Fused::NP;
+    if constexpr (TAB) fft_lds_wave<W, R, T, false, Fused::UNI, 0, 1, NPW - 1>(lds, WaveTable{ptab});
+    else fft_lds_wave<W, R, T, false, Fused::UNI, 0, 1, NPW - 1>(lds, tw);
+    __syncthreads();
+    // last forward pass (span QL): butterfly j of line r reads columns
+    // j + QL i and writes the same columns, element (a R + r) of the block:
+    // HBM index b0 W + t + T (s + PL i)
+    constexpr int RL = Fused::RL, QL = W / RL, TA = T / R, PL = QL / TA;
+    // an opaque copy of tid keeps the compiler from holding the load
+    // phase's LDS addresses in registers across the whole kernel
+    int t2 = tid;
+    asm volatile("" : "+v"(t2));
+    const int r = t2 % R, j0 = t2 / R;
+    const StreamOut out(A.x, A.n / 2);
+#pragma unroll
+    for (int s = 0; s < PL; s++) {
+      const int j = j0 + TA * s;
+      cplx x[RL];
+#pragma unroll
+      for (int i = 0; i < RL; i++) {
+        cplx a = lds[r * LP + lpad(j + QL * i)];
+        if (i > 0) {
+          cplx w;
+          if constexpr (TAB) w = ptab[QL - 1 + (i - 1) * QL + j];
+          else w = tw(i * j);
+          a = cmul(a, w);
+        }
+        x[i] = a;
+      }
+      dft<RL, false>(x);
+#pragma unroll
+      for (int i = 0; i < RL; i++) out.store(b0 * W + tid + T * (s + PL * i), x[i]);
+    }
+  } else {
+    fft_lds_auto<W, R, T, false, TAB>(lds, tw, ptab);
     constexpr int NE = (R * W + T - 1) / T;
     cplx v[NE];
     // an opaque copy of tid keeps the compiler from holding the load
