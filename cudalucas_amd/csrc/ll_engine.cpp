@@ -452,6 +452,9 @@ static int build_sweeps(ll_ctx *c) {
   A.q = p / n;
   A.b = p % n;
   A.remaining = nullptr;
+  A.stagger_sq = A.stagger_ca = 0;
+  if (c->sch.sk->h) A.stagger_sq = sweep_stagger(c->sch.sk->fn, c->sch.sk->threads, 0, W / 2, 32 * H, c->device);
+  if (c->sch.ck->w) A.stagger_ca = sweep_stagger(c->sch.ck->carry, c->sch.ck->threads, 0, H / R, 16 * R * W, c->device);
   if ((rc = dalloc(c, &c->d_args, 1))) return rc;
   if ((rc = dalloc(c, &c->d_args_unload, 1))) return rc;
   HIPCHK(c, hipMemcpyAsync(c->d_args, &A, sizeof(PlanArgs), hipMemcpyHostToDevice, c->stream));
@@ -1136,6 +1139,7 @@ int ll_batch_create(int32_t device, uint32_t count, const ll_batch_test *tests, 
         a[t].remaining = b->d_remaining + g.members[t];
         a[t].err = b->d_ring + (size_t)g.members[t] * kBatchRing;
         a[t].err_mask = kBatchRing - 1;
+        a[t].stagger_sq = a[t].stagger_ca = 0;  // several tests share the launch
       }
       if ((rc = balloc(b, &g.d_args, k))) return rc;
       BCHK(b, hipMemcpyAsync(g.d_args, a.data(), k * sizeof(PlanArgs), hipMemcpyHostToDevice, b->stream));

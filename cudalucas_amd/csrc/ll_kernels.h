@@ -68,6 +68,10 @@ struct PlanArgs {
   uint32_t rshift;         // log2(R): lines are tiled by R in memory
   uint32_t *remaining;     // iterations left (batched tests), or null
   uint32_t p, n, W, H, q, b;
+  // staggered start of the two-sweep kernels: workgroup i of the square
+  // (carry) sweep waits i * stagger_sq (stagger_ca) / 16 ticks of the
+  // 100 MHz real-time counter before its first load; 0: no wait
+  uint32_t stagger_sq, stagger_ca;
   GenLine gW, gH;          // plans of the generic kernels (unused otherwise)
 };
 
@@ -144,6 +148,15 @@ const SquareKernel *find_square(uint32_t H);
 const CarryKernel *find_carry(uint32_t W, uint32_t R);
 const CarryKernel *carry_table(size_t *count);
 const ResidentKernel *find_resident(uint32_t M);
+
+// Staggered-start step of a sweep kernel (PlanArgs::stagger_sq / _ca):
+// spreads the launch's loads over about the time the memory system needs
+// for them (bytes_per_group / 6144 sixteenths of a 10 ns tick per
+// workgroup, measured best at 1.5M-4M words) when all `groups` workgroups
+// run at once (one round); 0 when the grid runs in several rounds, where
+// later workgroups start as earlier ones finish anyway.
+uint32_t sweep_stagger(const void *fn, uint32_t threads, uint32_t dyn_lds, uint32_t groups,
+                       uint32_t bytes_per_group, int device);
 
 // Sweep launches over `tests` tests of one shape: d_args is a device array of
 // their PlanArgs, blockIdx.y selects the test.

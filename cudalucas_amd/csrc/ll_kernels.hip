@@ -242,6 +242,15 @@ const ResidentKernel *find_resident(uint32_t M) {
   return nullptr;
 }
 
+uint32_t sweep_stagger(const void *fn, uint32_t threads, uint32_t dyn_lds, uint32_t groups,
+                       uint32_t bytes_per_group, int device) {
+  int per_cu = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)threads, dyn_lds) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
+  if (per_cu <= 0 || (uint64_t)groups > (uint64_t)per_cu * (uint64_t)cus) return 0;
+  return bytes_per_group / 6144;
+}
+
 hipError_t launch_square(const SquareKernel *k, const PlanArgs *d_args, uint32_t tests, uint32_t W,
                          uint32_t H, hipStream_t s) {
   const PlanArgs *a = d_args;

@@ -49,6 +49,17 @@ constexpr uint32_t kSpinLimit = 1u << 22;
 
 // ------------------------------------------------------------------ helpers
 
+// Staggered start: workgroup i waits i * sixteenths / 16 ticks of the
+// 100 MHz real-time counter (measured from its own start) before its first
+// load, so that the sweep's loads arrive in workgroup order and early
+// workgroups compute and store while later ones load.
+__device__ __forceinline__ void stagger_start(uint32_t sixteenths) {
+  if (sixteenths) {
+    const uint64_t until = __builtin_amdgcn_s_memrealtime() + (((uint64_t)blockIdx.x * sixteenths) >> 4);
+    while (__builtin_amdgcn_s_memrealtime() < until) __builtin_amdgcn_s_sleep(1);
+  }
+}
+
 __device__ __forceinline__ cplx twiddle_m(const TwoLevel &t, uint32_t e) {
   const uint32_t k = 2 * (e >> t.shift);
   const cplx h = gload(t.hi + k), hr = gload(t.hi + k + 1);
@@ -168,6 +179,7 @@ __device__ __forceinline__ void pair_square(cplx &A, cplx &B, cplx t) {
 template <int H, int T>
 __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict__ args) {
   const PlanArgs &A = args[blockIdx.y];
+  stagger_start(A.stagger_sq);
   // Opens the iteration for the carry sweep that follows this launch
   // (workgroup 0) and tells whether this is a batched test that is done.
   // Called once the column loads are in flight, so that the state reads do
@@ -436,6 +448,7 @@ template <int W, int R, int T>
 __global__ __launch_bounds__(T, (T >= 512 ? 1 : carry_groups_per_cu<W, R, T>())) void carry_sweep(
     const PlanArgs *__restrict__ args) {
   const PlanArgs &A = args[blockIdx.y];
+  stagger_start(A.stagger_ca);
   constexpr int LP = lpitch(W);
   constexpr int CPT = (W + T - 1) / T;  // columns per thread
   __shared__ cplx lds[R * LP];
