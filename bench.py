@@ -24,6 +24,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
+# LL_OPT_NO_GRAPH under --no-graph: kernels launched one by one (for
+# profilers that do not follow hipGraph replay); the kernels are the same
+CTX_FLAGS = 0
 FP64_PEAK_TFLOPS = 78.6
 
 
@@ -164,7 +167,7 @@ def smooth_leg(iters=2000, warmup=100):
     rows = []
     for n in W.SMOOTH_LENGTHS:
         p = W.smooth_prime(n)
-        with L.LLContext(p, n) as c:
+        with L.LLContext(p, n, flags=CTX_FLAGS) as c:
             info = c.info()
             c.load_initial(0)
             c.iterate(warmup)
@@ -196,7 +199,7 @@ def lengths_leg(spec, iters=500, warmup=100):
     for tok in spec.split(","):
         n = L.fft_from_str(tok.strip())
         p = W.densest_prime(n)
-        with L.LLContext(p, n) as c:
+        with L.LLContext(p, n, flags=CTX_FLAGS) as c:
             info = c.info()
             c.load_initial(0)
             c.iterate(warmup)
@@ -255,7 +258,12 @@ def main():
                     help="batched leg without the one-exponent-at-a-time comparison")
     ap.add_argument("--no-smooth", action="store_true", help="skip the smooth-length and N=4096 legs")
     ap.add_argument("--lengths", default="", help="extra lengths to time, e.g. 8192K,3136K (densest exponents)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch each iteration's kernels one by one instead of replaying a hipGraph")
     args = ap.parse_args()
+    global CTX_FLAGS
+    if args.no_graph:
+        CTX_FLAGS = 2  # LL_OPT_NO_GRAPH
     if args.gpus != 1:
         print("bench.py: measures one GPU; the LL path of one exponent runs on a single "
               "MI355X, and several GPUs would each take their own exponents", file=sys.stderr)
@@ -277,7 +285,7 @@ def main():
     single_tuned = os.path.exists(T.SINGLE_TABLE)
     if single_tuned:  # schedules picked by the single-exponent autotuner
         T.install(T.load(T.SINGLE_TABLE))
-    ctx = L.LLContext(p, n)
+    ctx = L.LLContext(p, n, flags=CTX_FLAGS)
     if single_tuned:
         T.install([])
     info = ctx.info()

@@ -201,6 +201,10 @@ extern const CarryKernel k_carry_large[];
 extern const size_t k_carry_large_count;
 extern const CarryKernel k_carry_small[];
 extern const size_t k_carry_small_count;
+extern const CarryKernel k_carry_mixed[];
+extern const size_t k_carry_mixed_count;
+extern const CarryKernel k_carry_wide[];
+extern const size_t k_carry_wide_count;
 
 static const ResidentKernel k_resident[] = {
     LL_RESIDENT(256, 64),   LL_RESIDENT(512, 128),  LL_RESIDENT(1024, 256),
@@ -220,6 +224,8 @@ static const std::vector<CarryKernel> &all_carry() {
   static const std::vector<CarryKernel> v = [] {
     std::vector<CarryKernel> t(k_carry_large, k_carry_large + k_carry_large_count);
     t.insert(t.end(), k_carry_small, k_carry_small + k_carry_small_count);
+    t.insert(t.end(), k_carry_mixed, k_carry_mixed + k_carry_mixed_count);
+    t.insert(t.end(), k_carry_wide, k_carry_wide + k_carry_wide_count);
     return t;
   }();
   return v;
