@@ -32,7 +32,14 @@ constexpr int kGenWavesPerEU = 4;
 // One butterfly of a pass: forward (DIF) reads R values at stride s, takes
 // their length-R DFT and multiplies output k by w_L^(k j tstep); inverse
 // (DIT) multiplies input k by the conjugate, then takes the inverse DFT.
-template <int R, bool INV>
+// Roots of the plan: staged in LDS (LR) or read from global memory.
+template <bool LR>
+__device__ __forceinline__ cplx gen_root(const cplx *roots, int e) {
+  if constexpr (LR) return roots[e];
+  else return gload(roots + e);
+}
+
+template <int R, bool INV, bool LR>
 __device__ __forceinline__ void gen_bfly(cplx *buf, int base, int s, int jt, const cplx *__restrict__ roots) {
   cplx v[R];
 #pragma unroll
@@ -40,14 +47,14 @@ __device__ __forceinline__ void gen_bfly(cplx *buf, int base, int s, int jt, con
   if constexpr (INV) {
     if (jt) {
 #pragma unroll
-      for (int k = 1; k < R; k++) v[k] = cmulc(v[k], gload(roots + k * jt));
+      for (int k = 1; k < R; k++) v[k] = cmulc(v[k], gen_root<LR>(roots, k * jt));
     }
     dft<R, true>(v);
   } else {
     dft<R, false>(v);
     if (jt) {
 #pragma unroll
-      for (int k = 1; k < R; k++) v[k] = cmul(v[k], gload(roots + k * jt));
+      for (int k = 1; k < R; k++) v[k] = cmul(v[k], gen_root<LR>(roots, k * jt));
     }
   }
 #pragma unroll
@@ -55,8 +62,9 @@ __device__ __forceinline__ void gen_bfly(cplx *buf, int base, int s, int jt, con
 }
 
 // One pass of radix R and stride s over NB transforms at lds + b * LP.
-template <bool INV>
-__device__ __forceinline__ void gen_pass(cplx *lds, int LP, int NB, const GenLine &g, int R, int s) {
+template <bool INV, bool LR>
+__device__ __forceinline__ void gen_pass(cplx *lds, int LP, int NB, const GenLine &g, const cplx *roots, int R,
+                                         int s) {
   const int L = (int)g.L;
   const int Q = L / R;
   const int tstep = L / (R * s);
@@ -67,12 +75,12 @@ __device__ __forceinline__ void gen_pass(cplx *lds, int LP, int NB, const GenLin
     cplx *buf = lds + b * LP;
     const int jt = j * tstep;
     switch (R) {
-      case 2: gen_bfly<2, INV>(buf, base, s, jt, g.roots); break;
-      case 3: gen_bfly<3, INV>(buf, base, s, jt, g.roots); break;
-      case 4: gen_bfly<4, INV>(buf, base, s, jt, g.roots); break;
-      case 5: gen_bfly<5, INV>(buf, base, s, jt, g.roots); break;
-      case 7: gen_bfly<7, INV>(buf, base, s, jt, g.roots); break;
-      default: gen_bfly<8, INV>(buf, base, s, jt, g.roots); break;
+      case 2: gen_bfly<2, INV, LR>(buf, base, s, jt, roots); break;
+      case 3: gen_bfly<3, INV, LR>(buf, base, s, jt, roots); break;
+      case 4: gen_bfly<4, INV, LR>(buf, base, s, jt, roots); break;
+      case 5: gen_bfly<5, INV, LR>(buf, base, s, jt, roots); break;
+      case 7: gen_bfly<7, INV, LR>(buf, base, s, jt, roots); break;
+      default: gen_bfly<8, INV, LR>(buf, base, s, jt, roots); break;
     }
   }
 }
@@ -80,22 +88,22 @@ __device__ __forceinline__ void gen_pass(cplx *lds, int LP, int NB, const GenLin
 // Whole transforms of the NB lines; callers synchronise before.
 // Pass p has stride s_p = L / (r_0 ... r_p): forward passes run p = 0, 1,
 // ...; inverse passes run backwards from s = 1, s_(p-1) = s_p r_p.
-template <bool INV>
-__device__ __forceinline__ void gen_fft(cplx *lds, int LP, int NB, const GenLine &g) {
+template <bool INV, bool LR = false>
+__device__ __forceinline__ void gen_fft(cplx *lds, int LP, int NB, const GenLine &g, const cplx *roots) {
   const int np = (int)g.np;
   if constexpr (!INV) {
     int s = (int)g.L;
     for (int p = 0; p < np; p++) {
       const int r = g.radix[p];
       s /= r;
-      gen_pass<false>(lds, LP, NB, g, r, s);
+      gen_pass<false, LR>(lds, LP, NB, g, roots, r, s);
       __syncthreads();
     }
   } else {
     int s = 1;
     for (int p = np - 1; p >= 0; p--) {
       const int r = g.radix[p];
-      gen_pass<true>(lds, LP, NB, g, r, s);
+      gen_pass<true, LR>(lds, LP, NB, g, roots, r, s);
       __syncthreads();
       s *= r;
     }
@@ -141,9 +149,47 @@ __device__ __forceinline__ float gen_block_max(float v, float *red) {
 
 }  // namespace
 
+// Plans of at most kStage points keep their roots and digit-reversal table
+// in LDS beside the data (16 L + 4 L bytes), and those workgroups issue the
+// four-step twiddles of their pairing and store phases one transform ahead,
+// so that no pass or phase waits on a global read: the batched engine runs
+// thousands of such small workgroups, each bound by latency.
+constexpr int kStage = 1024;
+constexpr int kStagePer = kStage / kGenThreads;  // table entries per thread
+__host__ __device__ constexpr uint32_t stage_bytes(uint32_t L) { return L <= (uint32_t)kStage ? 20u * L : 0u; }
+
+// A plan's roots and inverse digit reversal on their way into LDS: read()
+// issues the global loads, write() stores them once they have arrived, so
+// that the signal loads go out in between.
+struct StagedPlan {
+  cplx r[kStagePer];
+  uint32_t q[kStagePer];
+  __device__ __forceinline__ void read(const GenLine &g) {
+#pragma unroll
+    for (int k = 0; k < kStagePer; k++) {
+      const int i = (int)threadIdx.x + k * kGenThreads;
+      if (i < (int)g.L) {
+        r[k] = gload(g.roots + i);
+        q[k] = gload(g.iperm + i);
+      }
+    }
+  }
+  __device__ __forceinline__ void write(const GenLine &g, cplx *rts, uint32_t *ipl) const {
+#pragma unroll
+    for (int k = 0; k < kStagePer; k++) {
+      const int i = (int)threadIdx.x + k * kGenThreads;
+      if (i < (int)g.L) {
+        rts[i] = r[k];
+        ipl[i] = q[k];
+      }
+    }
+  }
+};
+
 // --------------------------------------------------------- square sweep
 // W / 2 workgroups; workgroup g owns columns g and W - g (0 and W / 2 for
 // g = 0), as square_sweep.
+template <bool ST>
 __device__ __forceinline__ void square_generic_body(const PlanArgs &A, const uint32_t g, cplx *dyn) {
   const bool finished = A.remaining && *A.remaining == 0;
   if (g == 0 && threadIdx.x == 0) {  // column pair 0 opens the iteration
@@ -165,6 +211,10 @@ __device__ __forceinline__ void square_generic_body(const PlanArgs &A, const uin
   const uint32_t cols[2] = {g, g == 0 ? (uint32_t)W / 2 : (uint32_t)W - g};
   const uint32_t rs = A.rshift, rmask = (1u << rs) - 1;
   const int tid = threadIdx.x;
+  cplx *const rts = dyn + 2 * LP;                 // staged roots (ST)
+  uint32_t *const ipl = (uint32_t *)(rts + H);    // staged inverse digit reversal (ST)
+  StagedPlan sp;
+  if constexpr (ST) sp.read(A.gH);
   chunked(
       2 * H,
       [&](int i) {
@@ -177,17 +227,37 @@ __device__ __forceinline__ void square_generic_body(const PlanArgs &A, const uin
         const int which = i >= H, b = i - which * H;
         dyn[which * LP + lpad(b)] = cmul(v.a, v.b);
       });
+  if constexpr (ST) sp.write(A.gH, rts, ipl);
+  // the pairing twiddles of this thread, read while the transform runs
+  cplx tp[ST ? kStagePer : 1];
+  if constexpr (ST) {
+#pragma unroll
+    for (int k = 0; k < kStagePer; k++) {
+      const int k2 = tid + k * kGenThreads;
+      if (g != 0 && k2 < H) tp[k] = twiddle_m(A.twM, g + (uint32_t)W * (uint32_t)k2);
+    }
+  }
   __syncthreads();
-  gen_fft<false>(dyn, LP, 2, A.gH);
-  const uint32_t *ip = A.gH.iperm;
+  if constexpr (ST) gen_fft<false, true>(dyn, LP, 2, A.gH, rts);
+  else gen_fft<false>(dyn, LP, 2, A.gH, A.gH.roots);
+  const uint32_t *ip = ST ? ipl : A.gH.iperm;
   if (g != 0) {
     cplx *c0 = dyn, *c1 = dyn + LP;
-    for (int k2 = tid; k2 < H; k2 += blockDim.x) {
+    auto pair = [&](int k2, cplx t) {
       const int pa = lpad((int)ip[k2]), pb = lpad((int)ip[H - 1 - k2]);
       cplx a = c0[pa], b = c1[pb];
-      pair_square(a, b, twiddle_m(A.twM, g + (uint32_t)W * (uint32_t)k2));
+      pair_square(a, b, t);
       c0[pa] = a;
       c1[pb] = b;
+    };
+    if constexpr (ST) {
+#pragma unroll
+      for (int k = 0; k < kStagePer; k++) {
+        const int k2 = tid + k * kGenThreads;
+        if (k2 < H) pair(k2, tp[k]);
+      }
+    } else {
+      for (int k2 = tid; k2 < H; k2 += blockDim.x) pair(k2, twiddle_m(A.twM, g + (uint32_t)W * (uint32_t)k2));
     }
   } else {
     // column 0: k2 <-> H - k2 ; column W/2: k2 <-> H - 1 - k2
@@ -209,26 +279,49 @@ __device__ __forceinline__ void square_generic_body(const PlanArgs &A, const uin
       if (k2b != k2) c1[pb] = b;
     }
   }
-  __syncthreads();
-  gen_fft<true>(dyn, LP, 2, A.gH);
   const StreamOut out(A.x, A.n / 2);
-  chunked(
-      2 * H,
-      [&](int i) {
+  auto store = [&](int i, const cplx &w) {
+    const int which = i >= H, b = i - which * H;
+    const uint32_t c = cols[which];
+    const uint32_t off = (((uint32_t)b >> rs) * (uint32_t)W + c) << rs | ((uint32_t)b & rmask);
+    out.store(off, cmulc(dyn[which * LP + lpad(b)], w));
+  };
+  if constexpr (ST) {
+    // the store twiddles, read while the inverse transform runs
+    cplx ts[2 * kStagePer];
+#pragma unroll
+    for (int k = 0; k < 2 * kStagePer; k++) {
+      const int i = tid + k * kGenThreads;
+      if (i < 2 * H) {
         const int which = i >= H, b = i - which * H;
-        return twiddle_m(A.twM, cols[which] * (uint32_t)b);
-      },
-      [&](int i, const cplx &w) {
-        const int which = i >= H, b = i - which * H;
-        const uint32_t c = cols[which];
-        const uint32_t off = (((uint32_t)b >> rs) * (uint32_t)W + c) << rs | ((uint32_t)b & rmask);
-        out.store(off, cmulc(dyn[which * LP + lpad(b)], w));
-      });
+        ts[k] = twiddle_m(A.twM, cols[which] * (uint32_t)b);
+      }
+    }
+    __syncthreads();
+    gen_fft<true, true>(dyn, LP, 2, A.gH, rts);
+#pragma unroll
+    for (int k = 0; k < 2 * kStagePer; k++) {
+      const int i = tid + k * kGenThreads;
+      if (i < 2 * H) store(i, ts[k]);
+    }
+  } else {
+    __syncthreads();
+    gen_fft<true>(dyn, LP, 2, A.gH, A.gH.roots);
+    chunked(
+        2 * H,
+        [&](int i) {
+          const int which = i >= H, b = i - which * H;
+          return twiddle_m(A.twM, cols[which] * (uint32_t)b);
+        },
+        store);
+  }
 }
 
 __global__ __launch_bounds__(kGenThreads, kGenWavesPerEU) void square_generic(const PlanArgs *__restrict__ args) {
   extern __shared__ cplx dyn[];
-  square_generic_body(args[blockIdx.y], blockIdx.x, dyn);
+  const PlanArgs &A = args[blockIdx.y];
+  if (A.H <= (uint32_t)kStage) square_generic_body<true>(A, blockIdx.x, dyn);
+  else square_generic_body<false>(A, blockIdx.x, dyn);
 }
 
 // ---------------------------------------------------------- carry sweep
@@ -237,6 +330,7 @@ __global__ __launch_bounds__(kGenThreads, kGenWavesPerEU) void square_generic(co
 // column's 2R words, hand-off of the chain's carry-out to the next block
 // (epoch-tagged shuttle), carry-in through the whole chain, -2, reweight,
 // forward line transforms.  Balanced digits wait in LDS for the carry-in.
+template <bool ST>
 __device__ __forceinline__ void carry_generic_body(const PlanArgs &A, const uint32_t blk, cplx *dyn) {
   if (!A.st->run) return;  // a finished batched test
   const int W = (int)A.W, R = 1 << A.rshift, LP = lpitch(W);
@@ -246,12 +340,32 @@ __device__ __forceinline__ void carry_generic_body(const PlanArgs &A, const uint
   double *iwb = wb + 2 * R;
   int *fb = (int *)(iwb + 2 * R);
   float *red = (float *)(fb + 2 * R);
+  // staged roots and inverse digit reversal of the line plan (ST), after
+  // the block's own arrays, 16-byte aligned
+  cplx *const rts = lds + R * LP + (2 * R * (8 + 8 + 4) + (kGenThreads / 64) * 4 + 15) / 16;
+  uint32_t *const ipl = (uint32_t *)(rts + W);
   const int tid = threadIdx.x;
-  const uint32_t *ip = A.gW.iperm;
   const int rs = (int)A.rshift;
+  StagedPlan sp;
+  // per-column weights of this thread's columns a = tid + k T (ST)
+  int fa_c[ST ? kStagePer : 1];
+  double iwa_c[ST ? kStagePer : 1], wa_c[ST ? kStagePer : 1];
+  if constexpr (ST) {
+    sp.read(A.gW);
+#pragma unroll
+    for (int k = 0; k < kStagePer; k++) {
+      const int a = tid + k * kGenThreads;
+      if (a < W) {
+        fa_c[k] = gload(A.fA + a);
+        iwa_c[k] = gload(A.iwAs + a);
+        wa_c[k] = gload(A.wA + a);
+      }
+    }
+  }
+  const uint32_t *gip = A.gW.iperm;
   chunked(
       R * W,
-      [&](int e) { return CplxAt{gload_stream(A.x + (size_t)b0 * W + e), gload(ip + (e >> rs))}; },
+      [&](int e) { return CplxAt{gload_stream(A.x + (size_t)b0 * W + e), gload(gip + (e >> rs))}; },
       [&](int e, const CplxAt &v) { lds[(e & (R - 1)) * LP + lpad((int)v.pos)] = v.v; });
   if (tid < 2 * R) {
     const uint32_t i = 2 * b0 + tid;
@@ -259,19 +373,19 @@ __device__ __forceinline__ void carry_generic_body(const PlanArgs &A, const uint
     wb[tid] = gload(A.wB + i);
     iwb[tid] = gload(A.iwB + i);
   }
+  if constexpr (ST) sp.write(A.gW, rts, ipl);
   const uint32_t epoch = A.st->epoch;
   const uint32_t m2_digit = A.st->m2_digit;
   const double dmod = (double)A.st->m2_value;
   __syncthreads();
-  gen_fft<true>(lds, LP, R, A.gW);
+  if constexpr (ST) gen_fft<true, true>(lds, LP, R, A.gW, rts);
+  else gen_fft<true>(lds, LP, R, A.gW, A.gW.roots);
 
   const int q = (int)A.q;
   const int n = (int)A.n, bb = (int)A.b;
   double emax = 0.0;
   // round, subtract, carry along each column's chain; publish the carry-out
-  for (int a = tid; a < W; a += blockDim.x) {
-    const int fa = gload(A.fA + a);
-    const double iwa = gload(A.iwAs + a);
+  auto round_column = [&](const int a, const int fa, const double iwa) {
     const uint32_t m2i = m2_digit - 2u * ((uint32_t)a * H + b0);
     double carry = 0.0;
     for (int r = 0; r < R; r++) {
@@ -295,10 +409,10 @@ __device__ __forceinline__ void carry_generic_body(const PlanArgs &A, const uint
     const int32_t c32 = fits ? (int32_t)carry : 0;
     const uint64_t word = ((uint64_t)epoch << 32) | (uint32_t)c32;
     __hip_atomic_store(&A.shuttle[(size_t)blk * W + a], word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
+  };
   // carry-in from the previous block (block 0: the last block, one column
   // to the left), through the whole chain, then reweight
-  for (int a = tid; a < W; a += blockDim.x) {
+  auto carry_in_column = [&](const int a, const int fa, const double wa) {
     const size_t src = blk == 0 ? (size_t)(G - 1) * W + (a + W - 1) % W : (size_t)(blk - 1) * W + a;
     uint64_t word;
     uint32_t spins = 0;
@@ -312,8 +426,6 @@ __device__ __forceinline__ void carry_generic_body(const PlanArgs &A, const uint
       }
       __builtin_amdgcn_s_sleep(2);
     }
-    const int fa = gload(A.fA + a);
-    const double wa = gload(A.wA + a);
     double cc = (double)(int32_t)(uint32_t)word;
     for (int r = 0; r < R; r++) {
       const cplx dv = lds[r * LP + lpad(a)];
@@ -329,12 +441,29 @@ __device__ __forceinline__ void carry_generic_body(const PlanArgs &A, const uint
       }
       lds[r * LP + lpad(a)] = {x[0], x[1]};
     }
+  };
+  if constexpr (ST) {
+#pragma unroll
+    for (int k = 0; k < kStagePer; k++) {
+      const int a = tid + k * kGenThreads;
+      if (a < W) round_column(a, fa_c[k], iwa_c[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < kStagePer; k++) {
+      const int a = tid + k * kGenThreads;
+      if (a < W) carry_in_column(a, fa_c[k], wa_c[k]);
+    }
+  } else {
+    for (int a = tid; a < W; a += blockDim.x) round_column(a, gload(A.fA + a), gload(A.iwAs + a));
+    for (int a = tid; a < W; a += blockDim.x) carry_in_column(a, gload(A.fA + a), gload(A.wA + a));
   }
   __syncthreads();
-  gen_fft<false>(lds, LP, R, A.gW);
+  if constexpr (ST) gen_fft<false, true>(lds, LP, R, A.gW, rts);
+  else gen_fft<false>(lds, LP, R, A.gW, A.gW.roots);
   const StreamOut out(A.x, A.n / 2);
+  const uint32_t *ip = ST ? ipl : gip;
   chunked(
-      R * W, [&](int e) { return gload(ip + (e >> rs)); },
+      R * W, [&](int e) { return ip[e >> rs]; },
       [&](int e, const uint32_t &q) { out.store(b0 * (uint32_t)W + e, lds[(e & (R - 1)) * LP + lpad((int)q)]); });
   const float m = gen_block_max((float)emax, red);
   if (tid == 0) {
@@ -345,7 +474,9 @@ __device__ __forceinline__ void carry_generic_body(const PlanArgs &A, const uint
 
 __global__ __launch_bounds__(kGenThreads, kGenWavesPerEU) void carry_generic(const PlanArgs *__restrict__ args) {
   extern __shared__ cplx dyn[];
-  carry_generic_body(args[blockIdx.y], blockIdx.x, dyn);
+  const PlanArgs &A = args[blockIdx.y];
+  if (A.W <= (uint32_t)kStage) carry_generic_body<true>(A, blockIdx.x, dyn);
+  else carry_generic_body<false>(A, blockIdx.x, dyn);
 }
 
 // ------------------------------------------------- batched: one launch
@@ -369,7 +500,8 @@ __global__ __launch_bounds__(kGenThreads, kGenWavesPerEU) void batch_iteration(c
   const BatchItem it = items[blockIdx.x];
   const PlanArgs &A = args[it.test];
   if (!it.carry) {
-    square_generic_body(A, it.block, dyn);
+    if (A.H <= (uint32_t)kStage) square_generic_body<true>(A, it.block, dyn);
+    else square_generic_body<false>(A, it.block, dyn);
     __syncthreads();
     if (threadIdx.x == 0)
       __hip_atomic_fetch_add(&sq_done[it.test], 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
@@ -382,7 +514,10 @@ __global__ __launch_bounds__(kGenThreads, kGenWavesPerEU) void batch_iteration(c
     const unsigned long long want = (e + 1) * (A.W / 2);
     uint32_t spins = 0;
     int go = 1;
-    while (__hip_atomic_load(&sq_done[it.test], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < want) {
+    // relaxed polls: an acquire on every poll would invalidate this XCD's
+    // caches each time (buffer_inv) under every other workgroup; the one
+    // acquire fence below orders the item's reads after the wait
+    while (__hip_atomic_load(&sq_done[it.test], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < want) {
       if (++spins > kSpinLimit) {  // never hang: the iteration reports failure
         go = 0;
         break;
@@ -394,13 +529,16 @@ __global__ __launch_bounds__(kGenThreads, kGenWavesPerEU) void batch_iteration(c
   __syncthreads();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   if (s_go) {
-    carry_generic_body(A, it.block, dyn);
+    if (A.W <= (uint32_t)kStage) carry_generic_body<true>(A, it.block, dyn);
+    else carry_generic_body<false>(A, it.block, dyn);
   } else if (threadIdx.x == 0) {
     atomicMax(&A.err[(A.st->epoch - 1) & A.err_mask], __float_as_uint(3.0f));
   }
   __syncthreads();
+  // relaxed: ca_done only counts iterations for the waits above; what the
+  // carry items wrote is read in the next launch, after the kernel boundary
   if (threadIdx.x == 0)
-    __hip_atomic_fetch_add(&ca_done[it.test], 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(&ca_done[it.test], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 hipError_t launch_batch_iteration(const BatchItem *items, uint32_t count, const PlanArgs *args,
@@ -430,7 +568,7 @@ __global__ __launch_bounds__(kGenThreads, kGenWavesPerEU) void load_generic(cons
     }
   }
   __syncthreads();
-  gen_fft<false>(dyn, LP, R, A.gW);
+  gen_fft<false>(dyn, LP, R, A.gW, A.gW.roots);
   const uint32_t *ip = A.gW.iperm;
   for (int e = tid; e < R * W; e += blockDim.x) {
     const int r = e & (R - 1), a = e >> rs;
@@ -452,7 +590,7 @@ __global__ __launch_bounds__(kGenThreads, kGenWavesPerEU) void unload_generic(co
     dyn[r * LP + lpad((int)ip[a])] = A.x[(size_t)b0 * W + e];
   }
   __syncthreads();
-  gen_fft<true>(dyn, LP, R, A.gW);
+  gen_fft<true>(dyn, LP, R, A.gW, A.gW.roots);
   float emax = 0.f;
   const double scale = 1.0 / (double)W;
   for (int a = tid; a < W; a += blockDim.x) {
@@ -476,10 +614,13 @@ __global__ __launch_bounds__(kGenThreads, kGenWavesPerEU) void unload_generic(co
 
 // ------------------------------------------------------------- dispatch
 
-uint32_t square_generic_lds(uint32_t H) { return 2u * (uint32_t)lpitch((int)H) * (uint32_t)sizeof(cplx); }
+uint32_t square_generic_lds(uint32_t H) {
+  return 2u * (uint32_t)lpitch((int)H) * (uint32_t)sizeof(cplx) + stage_bytes(H);
+}
 
 uint32_t carry_generic_lds(uint32_t W, uint32_t R) {
-  return R * (uint32_t)lpitch((int)W) * (uint32_t)sizeof(cplx) + 2 * R * (8 + 8 + 4) + (kGenThreads / 64) * 4;
+  const uint32_t own = R * (uint32_t)lpitch((int)W) * (uint32_t)sizeof(cplx) + 2 * R * (8 + 8 + 4) + (kGenThreads / 64) * 4;
+  return stage_bytes(W) ? (own + 15) / 16 * 16 + stage_bytes(W) : own;
 }
 
 static const SquareKernel k_square_generic = {0, (const void *)square_generic, kGenThreads, nullptr};
