@@ -12,7 +12,7 @@ namespace llamd {
 extern const CarryKernel k_carry_large[] = {
     LL_CARRY(256, 4),  LL_CARRY(512, 4),  LL_CARRY(1024, 4), LL_CARRY(2048, 2),
     LL_CARRY(384, 4),  LL_CARRY(640, 4),  LL_CARRY(768, 4),  LL_CARRY(896, 4),
-    LL_CARRY(1280, 4), LL_CARRY(1536, 4), LL_CARRY(1792, 4), LL_CARRY(1024, 2),
+    LL_CARRY_T(1280, 4, 512), LL_CARRY_T(1536, 4, 512), LL_CARRY_T(1792, 4, 512), LL_CARRY(1024, 2),
     LL_CARRY(512, 8),  LL_CARRY_T(1024, 8, 512), LL_CARRY(256, 8), LL_CARRY(384, 8),
     LL_CARRY(640, 8),  LL_CARRY(768, 8), LL_CARRY_T(896, 8, 512),
     LL_CARRY_T(512, 16, 512),
