@@ -483,6 +483,25 @@ static int build_resident(ll_ctx *c) {
   int rc;
   if ((rc = upload(c, (double **)&D.w, w))) return rc;
   if ((rc = upload(c, (double **)&D.iw, iw))) return rc;
+  // factored weights for the iteration kernel's T threads of C words
+  const uint32_t T = c->sch.rk->threads, C = n / T;
+  std::vector<double> wT(T), iwT(T), iwTs(T), wC(C), iwC(C);
+  std::vector<int32_t> fT(T), fC(C);
+  for (uint32_t t = 0; t < T; t++) {
+    fT[t] = (int32_t)weight_exponent(p, n, (uint64_t)t * C);
+    weight_pair((uint32_t)fT[t], n, &wT[t], &iwT[t]);
+    iwTs[t] = inverse_weight_scaled((uint32_t)fT[t], n, 4ull * M);
+  }
+  for (uint32_t i = 0; i < C; i++) {
+    fC[i] = (int32_t)weight_exponent(p, n, i);
+    weight_pair((uint32_t)fC[i], n, &wC[i], &iwC[i]);
+  }
+  if ((rc = upload(c, (double **)&D.wT, wT))) return rc;
+  if ((rc = upload(c, (double **)&D.iwTs, iwTs))) return rc;
+  if ((rc = upload(c, (double **)&D.wC, wC))) return rc;
+  if ((rc = upload(c, (double **)&D.iwC, iwC))) return rc;
+  if ((rc = upload(c, (int32_t **)&D.fT, fT))) return rc;
+  if ((rc = upload(c, (int32_t **)&D.fC, fC))) return rc;
   if ((rc = upload(c, (double **)&D.qc, qc))) return rc;
   if ((rc = dalloc(c, &c->r_err, c->ring_size))) return rc;
   if ((rc = dalloc(c, &c->r_offset, 1))) return rc;

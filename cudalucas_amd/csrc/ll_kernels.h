@@ -77,8 +77,14 @@ struct PlanArgs {
 
 struct ResidentDesc {
   cplx *x;            // M complex, natural order (weighted words)
-  const double *w;    // 2^(f(j)/n)
-  const double *iw;   // 2^(-f(j)/n)
+  const double *w;    // 2^(f(j)/n)  (load / unload)
+  const double *iw;   // 2^(-f(j)/n) (load / unload)
+  // factored weights of the iteration kernel: word j = t C + i (thread t of
+  // T, C = n / T words each) has f(j) = fT[t] + fC[i] (mod n), weight
+  // wT[t] wC[i] (halved when the sum wraps) and unweight iwTs[t] iwC[i]
+  // (doubled when it wraps; iwTs includes the 1 / (4 M) of the round trip)
+  const double *wT, *iwTs, *wC, *iwC;
+  const int32_t *fT, *fC;
   const double *qc;   // cos(2 pi e/M), e <= M/4
   float *err;         // iters floats
   uint32_t *offset;   // shift offset, read at start and written at the end

@@ -34,6 +34,9 @@ __global__ __launch_bounds__(T) void resident_kernel(const ResidentDesc *descs) 
   __shared__ int64_t s_carry[T];
   __shared__ float red[T / 64];
   __shared__ Minus2 s_m2;
+  // per-word weight factors shared by every thread (word t C + i)
+  __shared__ double s_wC[C], s_iwC[C];
+  __shared__ int s_fC[C];
   const ResidentDesc D = descs[blockIdx.x];
   const uint32_t left = D.remaining ? *D.remaining : D.iters;
   const uint32_t iters = D.iters < left ? D.iters : left;
@@ -42,9 +45,17 @@ __global__ __launch_bounds__(T) void resident_kernel(const ResidentDesc *descs) 
   load_quarter<M, T>(qc, D.qc);
   const QuarterRoots<M> tw{qc};
   for (int i = tid; i < M; i += T) lds[lpad(i)] = D.x[i];
+  for (int i = tid; i < C; i += T) {
+    s_wC[i] = D.wC[i];
+    s_iwC[i] = D.iwC[i];
+    s_fC[i] = D.fC[i];
+  }
+  // this thread's factor, kept in registers for every iteration (the
+  // weights are never read from global memory inside the loop)
+  const double wT = D.wT[tid], iwTs = D.iwTs[tid];
+  const int fT = D.fT[tid];
   const uint32_t p = D.p, b = p % N, q = p / N;
   const uint32_t j0 = (uint32_t)tid * C;
-  const uint32_t f0 = (uint32_t)(((uint64_t)N - ((uint64_t)b * j0) % N) % N);
   uint32_t offset = *D.offset;
   __syncthreads();
   for (uint32_t it = 0; it < iters; it++) {
@@ -60,42 +71,54 @@ __global__ __launch_bounds__(T) void resident_kernel(const ResidentDesc *descs) 
     __syncthreads();
     fft_lds<M, 1, T, true>(lds, tw);
     const Minus2 m2 = s_m2;
-    int32_t dgt[C];
-    int widths[C];
+    // bit i: word i's weight exponent sum wrapped / word i is a big digit
+    uint64_t wraps = 0, bigs = 0;
+    static_assert(C <= 64, "per-word flags in one word");
     float emax = 0.f;
     int64_t carry = 0;
-    uint32_t f = f0;
+    // round, -2 and carry along this thread's C words; the balanced digits
+    // go back into the thread's own LDS slots (as exact doubles), so no
+    // per-word register array lives across the barrier below
 #pragma unroll
     for (int w = 0; w < C; w += 2) {
       const cplx v = lds[lpad((j0 + w) / 2)];
+      double dg[2];
 #pragma unroll
       for (int e = 0; e < 2; e++) {
         const uint32_t j = j0 + w + e;
-        const double t = (e ? v.y : v.x) * D.iw[j] * D.scale;
+        const int fs = fT + s_fC[w + e];
+        const int wrap = fs >= (int)N;
+        const int f = fs - (wrap ? (int)N : 0);
+        wraps |= (uint64_t)wrap << (w + e);
+        const double t = ldexp(mul3(e ? v.y : v.x, iwTs, s_iwC[w + e]), wrap);
         const double rt = rint(t);
         emax = fmaxf(emax, (float)fabs(t - rt));
         int64_t iv = (int64_t)rt;
         if (j == m2.digit) iv -= m2.value;
-        widths[w + e] = (int)q + (f < b ? 1 : 0);
-        dgt[w + e] = (int32_t)balance(iv + carry, widths[w + e], carry);
-        f = f >= b ? f - b : f + N - b;
+        const int big = f < (int)b;
+        bigs |= (uint64_t)big << (w + e);
+        dg[e] = (double)balance(iv + carry, (int)q + big, carry);
       }
+      lds[lpad((j0 + w) / 2)] = {dg[0], dg[1]};
     }
     s_carry[tid] = carry;
     __syncthreads();
     {
       // carry-in from the previous thread's words, propagated through this
       // thread's whole chain (see carry_sweep); the remainder stays in the
-      // last word
+      // last word; then reweight
       int64_t cc = s_carry[(tid + T - 1) % T];
 #pragma unroll
-      for (int w = 0; w < C; w++) dgt[w] = (int32_t)balance((int64_t)dgt[w] + cc, widths[w], cc);
-      dgt[C - 1] += (int32_t)(cc << widths[C - 1]);
-    }
-#pragma unroll
-    for (int w = 0; w < C; w += 2) {
-      const uint32_t j = j0 + w;
-      lds[lpad(j / 2)] = {(double)dgt[w] * D.w[j], (double)dgt[w + 1] * D.w[j + 1]};
+      for (int w = 0; w < C; w += 2) {
+        const cplx dv = lds[lpad((j0 + w) / 2)];
+        const int w0 = (int)q + (int)((bigs >> w) & 1), w1 = (int)q + (int)((bigs >> (w + 1)) & 1);
+        const int64_t d0 = balance((int64_t)dv.x + cc, w0, cc);
+        int64_t d1 = balance((int64_t)dv.y + cc, w1, cc);
+        if (w + 2 == C) d1 += cc << w1;
+        const double x0 = ldexp(mul3((double)d0, wT, s_wC[w]), -(int)((wraps >> w) & 1));
+        const double x1 = ldexp(mul3((double)d1, wT, s_wC[w + 1]), -(int)((wraps >> (w + 1)) & 1));
+        lds[lpad((j0 + w) / 2)] = {x0, x1};
+      }
     }
     const float m = block_max<T>(emax, red);
     if (tid == 0) D.err[it] = m;
