@@ -1,0 +1,21 @@
+// Carry-sweep / load / unload instantiations: lines of 2268..4096 points
+// (R = 2) beside 4096-point columns, for the 18144K..32768K
+// lengths of the reference's list whose line has a template.
+#include "ll_sweeps.h"
+
+namespace llamd {
+
+#define LL_CARRY_T(W, R, T)                                                              \
+  {W, R, (const void *)carry_sweep<W, R, T>, (const void *)load_lines<W, R, T>,          \
+   (const void *)unload_lines<W, R, T>, T,                                               \
+   carry_pass_table<W, R, T>() ? (const void *)line_pass_table<W, R, T> : nullptr}
+
+extern const CarryKernel k_carry_huge[] = {
+    LL_CARRY_T(2268, 2, 256), LL_CARRY_T(2500, 2, 256), LL_CARRY_T(2744, 2, 256),
+    LL_CARRY_T(2916, 2, 256), LL_CARRY_T(3136, 2, 256), LL_CARRY_T(3200, 2, 256),
+    LL_CARRY_T(3500, 2, 256), LL_CARRY_T(3584, 2, 256), LL_CARRY_T(3888, 2, 256),
+    LL_CARRY_T(4000, 2, 256), LL_CARRY_T(4096, 2, 256),
+};
+extern const size_t k_carry_huge_count = sizeof(k_carry_huge) / sizeof(k_carry_huge[0]);
+
+}  // namespace llamd

@@ -228,6 +228,10 @@ extern const CarryKernel k_carry_mixed[];
 extern const size_t k_carry_mixed_count;
 extern const CarryKernel k_carry_wide[];
 extern const size_t k_carry_wide_count;
+extern const CarryKernel k_carry_huge[];
+extern const size_t k_carry_huge_count;
+extern const CarryKernel k_carry_huge1[];
+extern const size_t k_carry_huge1_count;
 
 static const ResidentKernel k_resident[] = {
     LL_RESIDENT(256, 64),   LL_RESIDENT(512, 128),  LL_RESIDENT(1024, 256),
@@ -249,6 +253,8 @@ static const std::vector<CarryKernel> &all_carry() {
     t.insert(t.end(), k_carry_small, k_carry_small + k_carry_small_count);
     t.insert(t.end(), k_carry_mixed, k_carry_mixed + k_carry_mixed_count);
     t.insert(t.end(), k_carry_wide, k_carry_wide + k_carry_wide_count);
+    t.insert(t.end(), k_carry_huge, k_carry_huge + k_carry_huge_count);
+    t.insert(t.end(), k_carry_huge1, k_carry_huge1 + k_carry_huge1_count);
     return t;
   }();
   return v;
