@@ -15,7 +15,7 @@ extern const CarryKernel k_carry_large[] = {
     LL_CARRY_T(1280, 4, 512), LL_CARRY_T(1536, 4, 512), LL_CARRY_T(1792, 4, 512), LL_CARRY(1024, 2),
     LL_CARRY(512, 8),  LL_CARRY_T(1024, 8, 512), LL_CARRY(256, 8), LL_CARRY(384, 8),
     LL_CARRY(640, 8),  LL_CARRY(768, 8), LL_CARRY_T(896, 8, 512),
-    LL_CARRY_T(512, 16, 512),
+    LL_CARRY_T(512, 16, 512), LL_CARRY_T(2048, 4, 512),
 };
 extern const size_t k_carry_large_count = sizeof(k_carry_large) / sizeof(k_carry_large[0]);
 
