@@ -342,6 +342,48 @@ def test_length_coverage_vs_gmp_fixture(p, n, iters, gmp_fixtures):
     assert m < ROUNDOFF_LIMIT
 
 
+def _tuned_cases():
+    """Lengths whose autotuned schedule (the single-exponent table that
+    bench.py and the driver install) differs from the built-in preference
+    the coverage cases above run: the tuned schedule against the same GMP
+    fixture."""
+    from cudalucas_amd import tuner as T
+
+    with open(os.path.join(GOLDEN, "gmp_residues.json")) as f:
+        fx = json.load(f)
+    if not (os.path.exists(L.LIB_PATH) and os.path.exists(T.SINGLE_TABLE)):
+        return []
+    table = {s.n: s for s in T.load(T.SINGLE_TABLE)}
+    out = []
+    for n in O.default_fft_lengths():
+        e = table.get(n)
+        if n <= 16384 or e is None or e.path != 1 or not L.fft_supported(n):
+            continue
+        if L.fft_schedule(n)[1:] == (e.width, e.height, e.lines):
+            continue
+        p = W.densest_prime(n)
+        if str(p) not in fx:
+            continue
+        out.append(pytest.param(p, n, W.length_check_iters(p), e.width, e.height, e.lines,
+                                id="%dK-%dx%dx%d" % (n // 1024, e.width, e.height, e.lines)))
+    return out
+
+
+@pytest.mark.parametrize("p,n,iters,w,h,r", _tuned_cases())
+def test_tuned_schedule_vs_gmp_fixture(p, n, iters, w, h, r, gmp_fixtures):
+    announce("tuned schedule p=%d n=%d W=%d H=%d R=%d iters=%d" % (p, n, w, h, r, iters))
+    with L.LLContext(p, n, width=w, height=h, lines=r, flags=L.LL_OPT_FORCE_SWEEPS) as c:
+        info = c.info()
+        assert (info["width"], info["height"], info["lines"]) == (w, h, r)
+        c.load_initial(0)
+        m = c.iterate(iters)
+        wd, rr = c.residue()
+    fx = gmp_fixtures[(p, iters)]
+    assert "0x%016x" % rr == fx["res64"], info
+    assert sha(wd) == fx["sha256"], info
+    assert m < ROUNDOFF_LIMIT
+
+
 # --------------------------------------------------- per-call overhead
 
 def test_single_iteration_call_is_cheap():
