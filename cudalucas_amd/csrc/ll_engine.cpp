@@ -91,6 +91,7 @@ struct Schedule {
   const SquareKernel *sk;
   const CarryKernel *ck;
   bool generic;   // the runtime-plan kernels (ll_generic.hip)
+  uint32_t sq_grid;  // square-sweep workgroups along x (square_grid)
 };
 
 bool seven_smooth(uint32_t x) {
@@ -457,6 +458,7 @@ static int build_sweeps(ll_ctx *c) {
   A.stagger_sq = A.stagger_ca = 0;
   if (c->sch.sk->h) A.stagger_sq = sweep_stagger(c->sch.sk->fn, c->sch.sk->threads, 0, W / 2, 32 * H, c->device);
   if (c->sch.ck->w) A.stagger_ca = sweep_stagger(c->sch.ck->carry, c->sch.ck->threads, 0, H / R, 16 * R * W, c->device);
+  c->sch.sq_grid = square_grid(c->sch.sk, W, 1);
   if ((rc = dalloc(c, &c->d_args, 1))) return rc;
   if ((rc = dalloc(c, &c->d_args_unload, 1))) return rc;
   HIPCHK(c, hipMemcpyAsync(c->d_args, &A, sizeof(PlanArgs), hipMemcpyHostToDevice, c->stream));
@@ -540,7 +542,7 @@ static int build_graph(ll_ctx *c) {
   // relaxed: other threads may create contexts (and copy) while this captures
   HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
   for (uint32_t i = 0; i < kGraphIters; i++) {
-    (void)launch_square(c->sch.sk, c->d_args, 1, c->sch.W, c->sch.H, c->stream);
+    (void)launch_square(c->sch.sk, c->d_args, 1, c->sch.W, c->sch.H, c->sch.sq_grid, c->stream);
     (void)launch_carry(c->sch.ck, c->d_args, 1, c->sch.W, c->sch.H, c->sch.R, c->stream);
   }
   HIPCHK(c, hipStreamEndCapture(c->stream, &g));
@@ -694,7 +696,7 @@ static int iterate_sweeps(ll_ctx *c, uint32_t count, const PlanArgs *dargs, cons
     }
   }
   for (; left; left--) {
-    HIPCHK(c, launch_square(c->sch.sk, dargs, 1, c->sch.W, c->sch.H, c->stream));
+    HIPCHK(c, launch_square(c->sch.sk, dargs, 1, c->sch.W, c->sch.H, c->sch.sq_grid, c->stream));
     HIPCHK(c, launch_carry(c->sch.ck, dargs, 1, c->sch.W, c->sch.H, c->sch.R, c->stream));
   }
   HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
@@ -834,7 +836,7 @@ int ll_profile_kernels(ll_ctx *c, uint32_t count, float *kernel_ms) {
   GpuLock lock(c->device);
   for (uint32_t i = 0; i < count; i++) {
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-    HIPCHK(c, launch_square(c->sch.sk, c->d_args, 1, c->sch.W, c->sch.H, c->stream));
+    HIPCHK(c, launch_square(c->sch.sk, c->d_args, 1, c->sch.W, c->sch.H, c->sch.sq_grid, c->stream));
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
     HIPCHK(c, launch_carry(c->sch.ck, c->d_args, 1, c->sch.W, c->sch.H, c->sch.R, c->stream));
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
@@ -1029,7 +1031,8 @@ static int enqueue_sweeps(ll_batch *b, uint32_t iters) {
     for (auto &grp : b->groups)
       if (grp.sch.path == 1 && !grp.sch.generic) {
         const uint32_t k = (uint32_t)grp.members.size();
-        BCHK(b, launch_square(grp.sch.sk, grp.d_args, k, grp.sch.W, grp.sch.H, b->stream));
+        const uint32_t grid = square_grid(grp.sch.sk, grp.sch.W, k);
+        BCHK(b, launch_square(grp.sch.sk, grp.d_args, k, grp.sch.W, grp.sch.H, grid, b->stream));
         BCHK(b, launch_carry(grp.sch.ck, grp.d_args, k, grp.sch.W, grp.sch.H, grp.sch.R, b->stream));
       }
   }

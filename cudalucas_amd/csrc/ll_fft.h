@@ -309,13 +309,12 @@ __device__ __forceinline__ void load_quarter(double *dst, const double *__restri
 // w^(i*(j mod NS)) with w = exp(-+2 pi i/(NS*R)), applies a length-R DFT and
 // writes to (j/NS)*NS*R + (j mod NS) + u*NS.
 template <int L, int R, int NS, int NB, int T, bool INV, typename TW>
-__device__ __forceinline__ void stockham_pass(cplx *buf, const TW &tw) {
+__device__ __forceinline__ void stockham_pass(cplx *buf, const TW &tw, const int tid = threadIdx.x) {
   constexpr int Q = L / R;
   constexpr int NBF = NB * Q;
   constexpr int PER = (NBF + T - 1) / T;
   constexpr int LP = lpitch(L);
   cplx v[PER][R];
-  const int tid = threadIdx.x;
 #pragma unroll
   for (int s = 0; s < PER; s++) {
     const int u = tid + s * T;
@@ -563,13 +562,12 @@ struct PassTwiddles<L, NB, T, PASS, NS, false> {
 
 // stockham_pass with the thread's twiddles taken from a PassTwiddles level.
 template <int L, int R, int NS, int NB, int T, bool INV, typename PT>
-__device__ __forceinline__ void stockham_pass_cached(cplx *buf, const PT &pt) {
+__device__ __forceinline__ void stockham_pass_cached(cplx *buf, const PT &pt, const int tid = threadIdx.x) {
   constexpr int Q = L / R;
   constexpr int NBF = NB * Q;
   constexpr int PER = (NBF + T - 1) / T;
   constexpr int LP = lpitch(L);
   cplx v[PER][R];
-  const int tid = threadIdx.x;
 #pragma unroll
   for (int s = 0; s < PER; s++) {
     const int u = tid + s * T;
@@ -627,6 +625,68 @@ template <int K, typename PT>
 __device__ __forceinline__ const auto &twiddle_level(const PT &pt) {
   if constexpr (K == 0) return pt;
   else return twiddle_level<K - 1>(pt.next);
+}
+
+// Stockham pass of an NB x L transform whose twiddles w^(i k) are read from
+// LDS at tab[(i - 1) NS + k] (the layout of the per-pass table).
+template <int L, int R, int NS, int NB, int T, bool INV>
+__device__ __forceinline__ void stockham_pass_tab(cplx *buf, const cplx *tab, const int tid) {
+  constexpr int Q = L / R;
+  constexpr int NBF = NB * Q;
+  static_assert(NBF % T == 0, "stockham_pass_tab: whole butterflies per thread");
+  constexpr int PER = NBF / T;
+  constexpr int LP = lpitch(L);
+  cplx v[PER][R];
+#pragma unroll
+  for (int s = 0; s < PER; s++) {
+    const int u = tid + s * T;
+    const int b = u / Q, j = u % Q, k = j % NS;
+#pragma unroll
+    for (int i = 0; i < R; i++) {
+      cplx a = buf[b * LP + lpad(j + i * Q)];
+      if (i > 0) {
+        const cplx w = tab[(i - 1) * NS + k];
+        a = INV ? cmulc(a, w) : cmul(a, w);
+      }
+      v[s][i] = a;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < PER; s++) {
+    const int u = tid + s * T;
+    const int b = u / Q, j = u % Q, k = j % NS;
+    dft<R, INV>(v[s]);
+    const int base = (j / NS) * NS * R + k;
+#pragma unroll
+    for (int i = 0; i < R; i++) buf[b * LP + lpad(base + i * NS)] = v[s][i];
+  }
+  __syncthreads();
+}
+
+struct UnitRoots {
+  __device__ __forceinline__ cplx operator()(int) const { return {1.0, 0.0}; }
+};
+
+// A three-pass transform for a kernel that runs it in a loop: the first
+// pass has no twiddles, the second reads them from LDS (tab, its block of
+// the per-pass table), the third holds them in registers (pt2: the
+// PassTwiddles level of pass 2).  Each pass takes the thread index through
+// an empty asm statement, so that the compiler cannot hoist the passes'
+// LDS offsets out of the loop and keep them all live across it.
+template <int L, int NB, int T, bool INV, typename PT>
+__device__ __forceinline__ void fft_lds_3pass_in_loop(cplx *buf, const cplx *tab, const PT &pt2) {
+  constexpr int R0 = plan_radix(L, 0), R1 = plan_radix(L, 1), R2 = plan_radix(L, 2);
+  static_assert(R0 * R1 * R2 == L, "fft_lds_3pass_in_loop: a three-pass plan");
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
+  stockham_pass<L, R0, 1, NB, T, INV>(buf, UnitRoots{}, tid);
+  tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
+  stockham_pass_tab<L, R1, R0, NB, T, INV>(buf, tab, tid);
+  tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
+  stockham_pass_cached<L, R2, R0 * R1, NB, T, INV>(buf, pt2, tid);
 }
 
 }  // namespace llamd

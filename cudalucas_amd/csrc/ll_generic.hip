@@ -623,7 +623,7 @@ uint32_t carry_generic_lds(uint32_t W, uint32_t R) {
   return stage_bytes(W) ? (own + 15) / 16 * 16 + stage_bytes(W) : own;
 }
 
-static const SquareKernel k_square_generic = {0, (const void *)square_generic, kGenThreads, nullptr};
+static const SquareKernel k_square_generic = {0, (const void *)square_generic, kGenThreads, nullptr, nullptr};
 static const CarryKernel k_carry_generic = {0, 0, (const void *)carry_generic, (const void *)load_generic,
                                             (const void *)unload_generic, kGenThreads, nullptr};
 

@@ -98,6 +98,7 @@ struct SquareKernel {
   const void *fn;
   uint32_t threads;
   const void *table;  // builds the per-pass twiddles of the column plan
+  const void *pipe;   // persistent form of the same sweep (square_sweep_pipe), or null
 };
 
 struct CarryKernel {
@@ -168,8 +169,14 @@ uint32_t sweep_stagger(const void *fn, uint32_t threads, uint32_t dyn_lds, uint3
 // their PlanArgs, blockIdx.y selects the test.
 // W, H, R: the schedule's line length, column length and lines per carry
 // block (the generic kernels take them at run time).
+// Workgroups along x of a square-sweep launch of `tests` tests with W
+// columns on the current device: W / 2 (one per column pair), or one round
+// of the persistent form when k has one and W / 2 x tests pairs would take
+// more than one round of the plain kernel.
+uint32_t square_grid(const SquareKernel *k, uint32_t W, uint32_t tests);
+// grid: square_grid's value (the persistent form runs when grid < W / 2)
 hipError_t launch_square(const SquareKernel *k, const PlanArgs *d_args, uint32_t tests, uint32_t W,
-                         uint32_t H, hipStream_t s);
+                         uint32_t H, uint32_t grid, hipStream_t s);
 hipError_t launch_carry(const CarryKernel *k, const PlanArgs *d_args, uint32_t tests, uint32_t W,
                         uint32_t H, uint32_t R, hipStream_t s);
 hipError_t launch_load(const CarryKernel *k, const PlanArgs *d_args, uint32_t tests, uint32_t W,

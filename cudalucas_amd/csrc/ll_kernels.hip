@@ -14,6 +14,7 @@
 #include <stdint.h>
 
 #include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "ll_sweeps.h"
@@ -211,12 +212,15 @@ hipError_t noalloc_scratch(int device, void **ptr) {
 
 // ------------------------------------------------------------- dispatch
 
-#define LL_SQUARE(H) {H, (const void *)square_sweep<H, 256>, 256, (const void *)column_pass_table<H>}
-#define LL_SQUARE_T(H, T) {H, (const void *)square_sweep<H, T>, T, (const void *)column_pass_table<H>}
+#define LL_SQUARE(H) {H, (const void *)square_sweep<H, 256>, 256, (const void *)column_pass_table<H>, nullptr}
+#define LL_SQUARE_T(H, T) {H, (const void *)square_sweep<H, T>, T, (const void *)column_pass_table<H>, nullptr}
+#define LL_SQUARE_PIPE(H, T)                                                                           \
+  {H, (const void *)square_sweep<H, T>, T, (const void *)column_pass_table<H>,                         \
+   (const void *)square_sweep_pipe<H, T>}
 #define LL_RESIDENT(M, T) {M, (const void *)resident_kernel<M, T>, T}
 
 static const SquareKernel k_square[] = {
-    LL_SQUARE_T(128, 64), LL_SQUARE(256), LL_SQUARE(512), LL_SQUARE(1024), LL_SQUARE(2048), LL_SQUARE_T(4096, 512),
+    LL_SQUARE_T(128, 64), LL_SQUARE(256), LL_SQUARE(512), LL_SQUARE(1024), LL_SQUARE(2048), LL_SQUARE_PIPE(4096, 512),
 };
 
 // Carry-sweep instantiations live in ll_carry_*.hip (compiled in parallel).
@@ -286,12 +290,42 @@ uint32_t sweep_stagger(const void *fn, uint32_t threads, uint32_t dyn_lds, uint3
   return bytes_per_group / 6144;
 }
 
+// Workgroups of one round of a persistent sweep kernel on the current
+// device (its occupancy times the CUs), looked up once per kernel and device.
+static uint32_t one_round(const void *fn, uint32_t threads) {
+  static std::mutex mu;
+  static std::vector<std::tuple<const void *, int, uint32_t>> known;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  std::lock_guard<std::mutex> lock(mu);
+  for (const auto &e : known)
+    if (std::get<0>(e) == fn && std::get<1>(e) == dev) return std::get<2>(e);
+  int per_cu = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)threads, 0) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || per_cu <= 0 ||
+      cus <= 0)
+    return 0;
+  const uint32_t r = (uint32_t)per_cu * (uint32_t)cus;
+  known.emplace_back(fn, dev, r);
+  return r;
+}
+
+uint32_t square_grid(const SquareKernel *k, uint32_t W, uint32_t tests) {
+  if (!k->pipe || tests == 0) return W / 2;
+  const uint32_t plain = one_round(k->fn, k->threads);
+  if (plain == 0 || (uint64_t)(W / 2) * tests <= plain) return W / 2;
+  const uint32_t r = one_round(k->pipe, k->threads) / tests;
+  return r > 0 && r < W / 2 ? r : W / 2;
+}
+
 hipError_t launch_square(const SquareKernel *k, const PlanArgs *d_args, uint32_t tests, uint32_t W,
-                         uint32_t H, hipStream_t s) {
+                         uint32_t H, uint32_t grid, hipStream_t s) {
   const PlanArgs *a = d_args;
   void *args[] = {&a};
   const uint32_t lds = k->h ? 0 : square_generic_lds(H);
-  return hipLaunchKernel(k->fn, dim3(W / 2, tests), dim3(k->threads), args, lds, s);
+  if (grid == 0 || grid > W / 2) return hipErrorInvalidValue;
+  const void *fn = k->pipe && grid < W / 2 ? k->pipe : k->fn;
+  return hipLaunchKernel(fn, dim3(grid, tests), dim3(k->threads), args, lds, s);
 }
 
 hipError_t launch_carry(const CarryKernel *k, const PlanArgs *d_args, uint32_t tests, uint32_t W,

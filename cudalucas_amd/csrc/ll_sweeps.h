@@ -378,6 +378,135 @@ __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict_
   }
 }
 
+// ------------------------------------------------- persistent square sweep
+// For columns too long for two workgroups per CU (H = 4096: 139,776 bytes
+// of LDS), where a grid of W/2 column pairs runs in several rounds: a grid
+// of at most one round, each workgroup taking pairs g, g + gridDim.x, ...
+// and loading its next pair into registers while it transforms the
+// current one, so that the loads of the next pair and the stores of the
+// current one overlap the transforms instead of following them.  To leave
+// registers for the pair in flight, the second pass of the column
+// transforms reads its twiddles from a copy of its block of the per-pass
+// table in LDS instead of holding them in registers; otherwise the
+// arithmetic is square_sweep's (non-fused form) term for term, so the
+// results are the same integers.
+template <int H, int T>
+__global__ __launch_bounds__(T, 1) void square_sweep_pipe(const PlanArgs *__restrict__ args) {
+  const PlanArgs &A = args[blockIdx.y];
+  static_assert(H % T == 0, "square sweep: H must be a multiple of T");
+  constexpr int NE = H / T;
+  constexpr int LP = lpitch(H);
+  __shared__ cplx lds[2 * LP];
+  constexpr int S1 = plan_radix(H, 0), S2 = S1 * plan_radix(H, 1);
+  constexpr int NT1 = (plan_radix(H, 1) - 1) * S1;  // twiddles of the second pass
+  __shared__ cplx tw1[NT1];
+  const int tid = threadIdx.x;
+  const uint32_t W = A.W, npairs = W / 2;
+  const uint32_t rs = A.rshift, rmask = (1u << rs) - 1;
+  uint32_t g = blockIdx.x;
+  if (g >= npairs) return;
+  auto partner = [&](uint32_t gg) { return gg == 0 ? W / 2 : W - gg; };
+  cplx n0[NE], n1[NE];  // the pair being loaded
+  auto fetch = [&](uint32_t gg, int tid) {
+    const cplx *col0 = A.x + ((size_t)gg << rs), *col1 = A.x + ((size_t)partner(gg) << rs);
+#pragma unroll
+    for (int st = 0; st < NE; st++) {
+      const int i = tid + st * T;
+      const size_t off = ((size_t)(i >> rs) * W << rs) | (i & rmask);
+      n0[st] = gload_stream(col0 + off);
+      n1[st] = gload_stream(col1 + off);
+    }
+  };
+  fetch(g, tid);
+  // opens the iteration for the carry sweep that follows (workgroup 0, which
+  // takes pair 0 first), or stops a finished batched test
+  const bool finished = A.remaining && *A.remaining == 0;
+  if (blockIdx.x == 0 && tid == 0) {
+    DevState &st = *A.st;
+    if (finished) {
+      st.run = 0;
+    } else {
+      const Minus2 m2 = minus2_position(st.offset, A.p, A.n);
+      st.m2_digit = m2.digit;
+      st.m2_value = m2.value;
+      st.offset = m2.new_offset;
+      A.err[st.epoch & A.err_mask] = 0u;
+      st.epoch += 1;
+      st.run = 1;
+    }
+  }
+  if (finished) return;
+  for (int i = tid; i < NT1; i += T) tw1[i] = gload(A.ptH + (S1 - 1) + i);
+  PassTwiddles<H, 2, T, 2, S2> ptw;  // the third pass's, in registers
+  ptw.init_table(A.ptH);
+  const StreamOut out(A.x, A.n / 2);
+  for (;;) {
+    // the thread index, opaque to the compiler in each pass of the loop
+    // (as in fft_lds_3pass_in_loop)
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const uint32_t c0 = g, c1 = partner(g);
+    const uint32_t gn = g + gridDim.x;
+    {
+      const cplx t0 = twiddle_m(A.twM, c0 * (uint32_t)tid), t1 = twiddle_m(A.twM, c1 * (uint32_t)tid);
+#pragma unroll
+      for (int st = 0; st < NE; st++) {
+        const int i = tid + st * T;
+        const cplx w0 = cmul(t0, twiddle_m(A.twM, c0 * (uint32_t)(st * T)));
+        const cplx w1 = cmul(t1, twiddle_m(A.twM, c1 * (uint32_t)(st * T)));
+        lds[lpad(i)] = cmul(n0[st], w0);
+        lds[LP + lpad(i)] = cmul(n1[st], w1);
+      }
+    }
+    if (gn < npairs) fetch(gn, tid);
+    __syncthreads();
+    fft_lds_3pass_in_loop<H, 2, T, false>(lds, tw1, ptw);
+    if (g != 0) {
+#pragma unroll 4
+      for (int st = 0; st < H / T; st++) {
+        const int k2 = tid + st * T, k2b = H - 1 - k2;
+        cplx a = lds[lpad(k2)], b = lds[LP + lpad(k2b)];
+        pair_square(a, b, twiddle_m(A.twM, c0 + W * (uint32_t)k2));
+        lds[lpad(k2)] = a;
+        lds[LP + lpad(k2b)] = b;
+      }
+    } else {
+      // column 0: k2 <-> H - k2 ; column W/2: k2 <-> H - 1 - k2
+      for (int k2 = tid; k2 <= H / 2; k2 += T) {
+        const int k2b = (H - k2) % H;
+        cplx a = lds[lpad(k2)], b = lds[lpad(k2b)];
+        pair_square(a, b, twiddle_m(A.twM, W * (uint32_t)k2));
+        lds[lpad(k2)] = a;
+        if (k2b != k2) lds[lpad(k2b)] = b;
+      }
+      for (int k2 = tid; k2 < (H + 1) / 2; k2 += T) {
+        const int k2b = H - 1 - k2;
+        cplx a = lds[LP + lpad(k2)], b = lds[LP + lpad(k2b)];
+        pair_square(a, b, twiddle_m(A.twM, W / 2 + W * (uint32_t)k2));
+        lds[LP + lpad(k2)] = a;
+        if (k2b != k2) lds[LP + lpad(k2b)] = b;
+      }
+    }
+    __syncthreads();
+    fft_lds_3pass_in_loop<H, 2, T, true>(lds, tw1, ptw);
+    {
+      const cplx t0 = twiddle_m(A.twM, c0 * (uint32_t)tid), t1 = twiddle_m(A.twM, c1 * (uint32_t)tid);
+#pragma unroll 2
+      for (int st = 0; st < H / T; st++) {
+        const int i = tid + st * T;
+        const uint32_t off = (((uint32_t)i >> rs) * W << rs) | (i & rmask);
+        const cplx w0 = cmul(t0, twiddle_m(A.twM, c0 * (uint32_t)(st * T)));
+        const cplx w1 = cmul(t1, twiddle_m(A.twM, c1 * (uint32_t)(st * T)));
+        out.store((c0 << rs) + off, cmulc(lds[lpad(i)], w0));
+        out.store((c1 << rs) + off, cmulc(lds[LP + lpad(i)], w1));
+      }
+    }
+    if (gn >= npairs) break;
+    g = gn;
+    __syncthreads();  // every thread has read this pair from LDS
+  }
+}
+
 // ------------------------------------------------------------- carry sweep
 // H/R workgroups, workgroup k owning block k of R lines.  Each publishes the
 // carries out of its block before it waits for those into it, so a wait is
