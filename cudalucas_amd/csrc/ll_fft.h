@@ -348,13 +348,20 @@ __device__ __forceinline__ void stockham_pass(cplx *buf, const TW &tw, const int
   __syncthreads();
 }
 
-template <int L, int NB, int T, bool INV, int PASS = 0, int NS = 1, int END = -1, typename TW>
+// OPQ: as in fft_lds_wave.
+template <int L, int NB, int T, bool INV, int PASS = 0, int NS = 1, int END = -1, bool OPQ = false, typename TW>
 __device__ __forceinline__ void fft_lds(cplx *buf, const TW &tw) {
   if constexpr (NS < L && PASS != END) {
     constexpr int R = plan_radix(L, PASS);
     static_assert(R > 0, "bad plan");
-    stockham_pass<L, R, NS, NB, T, INV>(buf, tw);
-    fft_lds<L, NB, T, INV, PASS + 1, NS * R, END>(buf, tw);
+    if constexpr (OPQ) {
+      int t = threadIdx.x;
+      asm volatile("" : "+v"(t));
+      stockham_pass<L, R, NS, NB, T, INV>(buf, tw, t);
+    } else {
+      stockham_pass<L, R, NS, NB, T, INV>(buf, tw);
+    }
+    fft_lds<L, NB, T, INV, PASS + 1, NS * R, END, OPQ>(buf, tw);
   }
 }
 
@@ -437,13 +444,13 @@ __device__ __forceinline__ void build_wave_table(cplx *pt, const TW &tw) {
 }
 
 template <int L, int R, int NS, int NB, int T, bool INV, typename TW>
-__device__ __forceinline__ void stockham_pass_wave(cplx *buf, const TW &tw) {
+__device__ __forceinline__ void stockham_pass_wave(cplx *buf, const TW &tw, const int tid = threadIdx.x) {
   constexpr int Q = L / R;
   constexpr int LW = NB / (T / 64);
   constexpr int PER = LW * Q / 64;
   constexpr int LP = lpitch(L);
   cplx v[PER][R];
-  const int lane = threadIdx.x & 63, line0 = (threadIdx.x >> 6) * LW;
+  const int lane = tid & 63, line0 = (tid >> 6) * LW;
 #pragma unroll
   for (int s = 0; s < PER; s++) {
     const int u = lane + 64 * s;
@@ -480,15 +487,24 @@ __device__ __forceinline__ void stockham_pass_wave(cplx *buf, const TW &tw) {
 
 // NB transforms of length L, each wave transforming its own lines; callers
 // synchronise the workgroup before and after.  Passes PASS .. END - 1 are
-// run (END = -1: to the last pass).
+// run (END = -1: to the last pass).  OPQ: each pass takes the thread index
+// through an empty asm statement, for kernels that run the transform in a
+// loop (the compiler cannot then hoist every pass's LDS offsets out of the
+// loop and keep them all live across it).
 template <int L, int NB, int T, bool INV, bool UNI = wave_plan_ok<L, NB, T, true>(), int PASS = 0,
-          int NS = 1, int END = -1, typename TW>
+          int NS = 1, int END = -1, bool OPQ = false, typename TW>
 __device__ __forceinline__ void fft_lds_wave(cplx *buf, const TW &tw) {
   static_assert(wave_plan_ok<L, NB, T, UNI>(), "fft_lds_wave: plan is not wave-local");
   if constexpr (NS < L && PASS != END) {
     constexpr int R = wave_radix<UNI>(L, PASS);
-    stockham_pass_wave<L, R, NS, NB, T, INV>(buf, tw);
-    fft_lds_wave<L, NB, T, INV, UNI, PASS + 1, NS * R, END>(buf, tw);
+    if constexpr (OPQ) {
+      int t = threadIdx.x;
+      asm volatile("" : "+v"(t));
+      stockham_pass_wave<L, R, NS, NB, T, INV>(buf, tw, t);
+    } else {
+      stockham_pass_wave<L, R, NS, NB, T, INV>(buf, tw);
+    }
+    fft_lds_wave<L, NB, T, INV, UNI, PASS + 1, NS * R, END, OPQ>(buf, tw);
   }
 }
 
@@ -503,15 +519,16 @@ constexpr int wave_passes(int L) {
 // Either form, chosen at compile time.  With TAB the wave-local form takes
 // its twiddles from the per-pass table `pt` (build_wave_table of the plan
 // auto_plan_uniform selects, L - 1 entries), else from `tw`.
-template <int L, int NB, int T, bool INV, bool TAB, typename TW>
+template <int L, int NB, int T, bool INV, bool TAB, bool OPQ = false, typename TW>
 __device__ __forceinline__ void fft_lds_auto(cplx *buf, const TW &tw, const cplx *pt) {
   if constexpr (wave_local_ok<L, NB, T>()) {
+    constexpr bool UNI = wave_plan_ok<L, NB, T, true>();
     __syncthreads();
-    if constexpr (TAB) fft_lds_wave<L, NB, T, INV>(buf, WaveTable{pt});
-    else fft_lds_wave<L, NB, T, INV>(buf, tw);
+    if constexpr (TAB) fft_lds_wave<L, NB, T, INV, UNI, 0, 1, -1, OPQ>(buf, WaveTable{pt});
+    else fft_lds_wave<L, NB, T, INV, UNI, 0, 1, -1, OPQ>(buf, tw);
     __syncthreads();
   } else {
-    fft_lds<L, NB, T, INV>(buf, tw);
+    fft_lds<L, NB, T, INV, 0, 1, -1, OPQ>(buf, tw);
   }
 }
 

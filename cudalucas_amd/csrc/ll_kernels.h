@@ -106,6 +106,7 @@ struct CarryKernel {
   const void *carry, *load, *unload;
   uint32_t threads;
   const void *table;  // builds the line pass table ptW (null: not used)
+  const void *pipe;   // persistent form of the carry sweep (carry_sweep_pipe), or null
 };
 
 // The generic square / carry kernels (ll_generic.hip): one instantiation
@@ -177,8 +178,14 @@ uint32_t square_grid(const SquareKernel *k, uint32_t W, uint32_t tests);
 // grid: square_grid's value (the persistent form runs when grid < W / 2)
 hipError_t launch_square(const SquareKernel *k, const PlanArgs *d_args, uint32_t tests, uint32_t W,
                          uint32_t H, uint32_t grid, hipStream_t s);
+// Workgroups along x of a carry-sweep launch of `tests` tests with H / R
+// line blocks each on the current device: H / R, or one round of the
+// persistent form when k has one and the blocks would take more than one
+// round of the plain kernel.
+uint32_t carry_grid(const CarryKernel *k, uint32_t H, uint32_t R, uint32_t tests);
+// grid: carry_grid's value (the persistent form runs when grid < H / R)
 hipError_t launch_carry(const CarryKernel *k, const PlanArgs *d_args, uint32_t tests, uint32_t W,
-                        uint32_t H, uint32_t R, hipStream_t s);
+                        uint32_t H, uint32_t R, uint32_t grid, hipStream_t s);
 hipError_t launch_load(const CarryKernel *k, const PlanArgs *d_args, uint32_t tests, uint32_t W,
                        uint32_t H, uint32_t R, hipStream_t s);
 hipError_t launch_unload(const CarryKernel *k, const PlanArgs *d_args, uint32_t tests, uint32_t W,

@@ -328,12 +328,23 @@ hipError_t launch_square(const SquareKernel *k, const PlanArgs *d_args, uint32_t
   return hipLaunchKernel(fn, dim3(grid, tests), dim3(k->threads), args, lds, s);
 }
 
+uint32_t carry_grid(const CarryKernel *k, uint32_t H, uint32_t R, uint32_t tests) {
+  const uint32_t G = R ? H / R : 0;
+  if (!k->pipe || tests == 0) return G;
+  const uint32_t plain = one_round(k->carry, k->threads);
+  if (plain == 0 || (uint64_t)G * tests <= plain) return G;
+  const uint32_t r = one_round(k->pipe, k->threads) / tests;
+  return r > 0 && r < G ? r : G;
+}
+
 hipError_t launch_carry(const CarryKernel *k, const PlanArgs *d_args, uint32_t tests, uint32_t W,
-                        uint32_t H, uint32_t R, hipStream_t s) {
+                        uint32_t H, uint32_t R, uint32_t grid, hipStream_t s) {
   const PlanArgs *a = d_args;
   void *args[] = {&a};
   const uint32_t lds = k->w ? 0 : carry_generic_lds(W, R);
-  return hipLaunchKernel(k->carry, dim3(H / R, tests), dim3(k->threads), args, lds, s);
+  if (grid == 0 || grid > H / R) return hipErrorInvalidValue;
+  const void *fn = k->pipe && grid < H / R ? k->pipe : k->carry;
+  return hipLaunchKernel(fn, dim3(grid, tests), dim3(k->threads), args, lds, s);
 }
 
 hipError_t launch_load(const CarryKernel *k, const PlanArgs *d_args, uint32_t tests, uint32_t W,

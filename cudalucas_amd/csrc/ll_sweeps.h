@@ -856,6 +856,313 @@ __global__ __launch_bounds__(T, (T >= 512 ? 1 : carry_groups_per_cu<W, R, T>()))
   }
 }
 
+// ------------------------------------------------- persistent carry sweep
+// For lengths whose H / R line blocks take more than one round of
+// carry_sweep at T >= 512 (one workgroup per CU): one round of workgroups,
+// workgroup w taking blocks w, w + gridDim.x, ... and loading its next
+// block into registers while it works on the current one, so that the
+// loads of the next block and the stores of the current one overlap the
+// transforms and the carries.  The arithmetic is carry_sweep's term for
+// term.  Every wait ends: a block publishes its own carries before it
+// waits for its predecessor's, so the chain of waits behind any block
+// steps back one block and one pass over the grid at a time and ends at a
+// block that waits on nothing.  The exception is block 0, whose carry-in
+// comes from the last block: workgroup 0 would hold it (and so its later
+// blocks, and the blocks after those) for the whole sweep.  So workgroup 0
+// publishes block 0's carries first, parks its balanced digits in block 0's
+// own words of the signal (which no other workgroup reads or writes in
+// this launch), and finishes the block -- carry-in, reweight, forward
+// transform, store -- after its last other block.
+template <int W, int R, int T>
+__global__ __launch_bounds__(T, 1) void carry_sweep_pipe(const PlanArgs *__restrict__ args) {
+  static_assert(T >= 512, "carry_sweep_pipe keeps the per-column weights in registers");
+  const PlanArgs &A = args[blockIdx.y];
+  constexpr int LP = lpitch(W);
+  constexpr int CPT = (W + T - 1) / T;     // columns per thread
+  constexpr int NE = (R * W + T - 1) / T;  // block elements per thread
+  constexpr bool TAB = carry_pass_table<W, R, T>();
+  using Fused = CarryEnds<W, R, T>;
+  constexpr bool FUSE = Fused::ok;
+  __shared__ cplx lds[R * LP];
+  __shared__ double qc[TAB ? 1 : W / 4 + 1];
+  __shared__ int fb[2 * R];
+  __shared__ double wb[2 * R], iwb[2 * R];
+  __shared__ float red[T / 64];
+  __shared__ cplx ptab[TAB ? W : 1];
+  const uint32_t H = A.H;
+  const uint32_t G = H / R;
+  const uint32_t first = blockIdx.x, stride = gridDim.x;
+  if (first >= G) return;
+  const QuarterRoots<W> tw{qc};
+  // thread t holds elements e = t + s T of a block: line e mod R, column e / R
+  cplx v[NE];
+  auto fetch = [&](uint32_t blk, int t) {
+    const cplx *src = A.x + (size_t)blk * R * W;
+#pragma unroll
+    for (int s = 0; s < NE; s++) {
+      const int e = t + s * T;
+      if ((R * W) % T == 0 || e < R * W) v[s] = gload_stream(src + e);
+    }
+  };
+  fetch(first, threadIdx.x);
+  if (!A.st->run) return;  // a finished batched test
+  const int tid0 = threadIdx.x;
+  if constexpr (TAB) {
+    for (int i = tid0; i < W - 1; i += T) ptab[i] = gload(A.ptW + i);
+  } else {
+    load_quarter<W, T>(qc, A.qcW);
+  }
+  int fa_c[CPT];
+  double iwa_c[CPT], wa_c[CPT];
+#pragma unroll
+  for (int c = 0; c < CPT; c++) {
+    const int a = tid0 + c * T;
+    if (W % T == 0 || a < W) {
+      fa_c[c] = gload(A.fA + a);
+      iwa_c[c] = gload(A.iwAs + a);
+      wa_c[c] = gload(A.wA + a);
+    }
+  }
+  const uint32_t epoch = A.st->epoch;
+  const uint32_t m2_digit = A.st->m2_digit;
+  const double dmod = (double)A.st->m2_value;
+  const int q = (int)A.q;
+  double emax = 0.0;
+  // the balanced digits go back into their column's own LDS slots until the
+  // carry-in arrives (registers hold the next block's loads)
+  uint64_t flags[CPT];  // wrap and big flags of each word (bits 2i, 2i + 1)
+  static_assert(2 * R <= 32, "two flag bits per word of the chain");
+  const StreamOut out(A.x, A.n / 2);
+
+  // the weight factors of block blk's rows, shared by every column
+  auto row_weights = [&](uint32_t blk, int t) {
+    if (t < 2 * R) {
+      const uint32_t i = 2 * blk * R + t;
+      fb[t] = gload(A.fB + i);
+      wb[t] = gload(A.wB + i);
+      iwb[t] = gload(A.iwB + i);
+    }
+  };
+  // round, subtract, carry along column a's 2R words; publish the carry-out
+  auto round_column = [&](const uint32_t blk, const int a, const int c) {
+    const uint32_t b0 = blk * R;
+    const uint32_t m2i = m2_digit - 2u * ((uint32_t)a * H + b0);
+    double carry = 0.0;
+    uint64_t fl = 0;
+    auto chain = [&](auto with_m2) {
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const cplx x = lds[r * LP + lpad(a)];
+        double d[2];
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+          const int i = 2 * r + e;
+          const int f0 = fa_c[c] + fb[i];
+          const int wrap = f0 >= (int)A.n;
+          const int big = f0 - (wrap ? (int)A.n : 0) < (int)A.b;
+          fl |= ((uint64_t)wrap | ((uint64_t)big << 1)) << (2 * i);
+          const double t = ldexp(mul3(e ? x.y : x.x, iwa_c[c], iwb[i]), wrap);
+          double rt = rint(t);
+          emax = fmax(emax, fabs(t - rt));
+          if constexpr (decltype(with_m2)::value) rt -= (uint32_t)i == m2i ? dmod : 0.0;
+          d[e] = balance_f64(rt + carry, q + big, carry);
+        }
+        lds[r * LP + lpad(a)] = {d[0], d[1]};
+      }
+    };
+    if (m2i < 2u * R) chain(std::true_type{});
+    else chain(std::false_type{});
+    flags[c] = fl;
+    const bool fits = fabs(carry) < 2147483648.0;
+    if (!fits) emax = fmax(emax, 2.0);  // carry overflow: flagged as a failed iteration
+    const int32_t c32 = fits ? (int32_t)carry : 0;
+    const uint64_t word = ((uint64_t)epoch << 32) | (uint32_t)c32;
+    __hip_atomic_store(&A.shuttle[(size_t)blk * W + a], word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  };
+  // carry-in from the previous block (block 0: the last block, one column
+  // to the left) through the whole chain, then reweight into LDS
+  auto carry_in_column = [&](const uint32_t blk, const int a, const int c) {
+    const size_t src = blk == 0 ? (size_t)(G - 1) * W + (a + W - 1) % W : (size_t)(blk - 1) * W + a;
+    uint64_t word;
+    uint32_t spins = 0;
+    for (;;) {
+      word = __hip_atomic_load(&A.shuttle[src], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if ((uint32_t)(word >> 32) == epoch) break;
+      if (++spins > kSpinLimit) {  // never hang: flag the iteration as failed
+        emax = fmax(emax, 3.0);
+        word = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    double cc = (double)(int32_t)(uint32_t)word;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const cplx dv = lds[r * LP + lpad(a)];
+      double x[2];
+#pragma unroll
+      for (int e = 0; e < 2; e++) {
+        const int i = 2 * r + e;
+        const int wrap = (int)(flags[c] >> (2 * i)) & 1;
+        const int width = q + ((int)(flags[c] >> (2 * i + 1)) & 1);
+        double d = balance_f64((e ? dv.y : dv.x) + cc, width, cc);
+        if (i == 2 * R - 1) d += ldexp(cc, width);
+        x[e] = ldexp(mul3(d, wa_c[c], wb[i]), -wrap);
+      }
+      lds[r * LP + lpad(a)] = {x[0], x[1]};
+    }
+  };
+  // forward transform of the block in LDS and its store to HBM
+  auto forward_store = [&](const uint32_t blk) {
+    const uint32_t b0 = blk * R;
+    if constexpr (FUSE) {
+      constexpr int NPW = Fused::NP;
+      if constexpr (TAB) fft_lds_wave<W, R, T, false, Fused::UNI, 0, 1, NPW - 1, true>(lds, WaveTable{ptab});
+      else fft_lds_wave<W, R, T, false, Fused::UNI, 0, 1, NPW - 1, true>(lds, tw);
+      __syncthreads();
+      constexpr int RL = Fused::RL, QL = W / RL, TA = T / R, PL = QL / TA;
+      int t2 = threadIdx.x;
+      asm volatile("" : "+v"(t2));
+      const int r = t2 % R, j0 = t2 / R;
+#pragma unroll
+      for (int s = 0; s < PL; s++) {
+        const int j = j0 + TA * s;
+        cplx x[RL];
+#pragma unroll
+        for (int i = 0; i < RL; i++) {
+          cplx a = lds[r * LP + lpad(j + QL * i)];
+          if (i > 0) {
+            cplx w;
+            if constexpr (TAB) w = ptab[QL - 1 + (i - 1) * QL + j];
+            else w = tw(i * j);
+            a = cmul(a, w);
+          }
+          x[i] = a;
+        }
+        dft<RL, false>(x);
+#pragma unroll
+        for (int i = 0; i < RL; i++) out.store(b0 * W + t2 + T * (s + PL * i), x[i]);
+      }
+    } else {
+      fft_lds_auto<W, R, T, false, TAB, true>(lds, tw, ptab);
+      int t2 = threadIdx.x;
+      asm volatile("" : "+v"(t2));
+#pragma unroll
+      for (int s = 0; s < NE; s++) {
+        const int e = t2 + s * T;
+        if ((R * W) % T == 0 || e < R * W) out.store(b0 * W + e, lds[(e % R) * LP + lpad(e / R)]);
+      }
+    }
+  };
+
+  for (uint32_t blk = first;;) {
+    // the thread index, opaque to the compiler in each pass of the loop, so
+    // that the passes' LDS offsets are not all hoisted out of it
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    const uint32_t next = blk + stride;
+    __syncthreads();  // the previous block's LDS reads are done
+    row_weights(blk, t);
+    // the block into LDS; when fused, the first inverse pass runs in
+    // registers on the way (butterfly j of line r takes columns j + Q0 i,
+    // which this thread holds as v[s0 + M0 i])
+    if constexpr (FUSE) {
+      constexpr int R0 = Fused::R0, M0 = Fused::M0, TA = T / R;
+      const int r = t % R, j0 = t / R;
+#pragma unroll
+      for (int s0 = 0; s0 < M0; s0++) {
+        cplx x[R0];
+#pragma unroll
+        for (int i = 0; i < R0; i++) x[i] = v[s0 + M0 * i];
+        dft<R0, true>(x);
+        const int j = j0 + TA * s0;
+#pragma unroll
+        for (int i = 0; i < R0; i++) lds[r * LP + lpad(j * R0 + i)] = x[i];
+      }
+    } else {
+#pragma unroll
+      for (int s = 0; s < NE; s++) {
+        const int e = t + s * T;
+        if ((R * W) % T == 0 || e < R * W) lds[(e % R) * LP + lpad(e / R)] = v[s];
+      }
+    }
+    if (next < G) fetch(next, t);
+    __syncthreads();
+    if constexpr (FUSE) {
+      if constexpr (TAB) fft_lds_wave<W, R, T, true, Fused::UNI, 1, Fused::R0, -1, true>(lds, WaveTable{ptab});
+      else fft_lds_wave<W, R, T, true, Fused::UNI, 1, Fused::R0, -1, true>(lds, tw);
+      __syncthreads();
+    } else {
+      fft_lds_auto<W, R, T, true, TAB, true>(lds, tw, ptab);
+    }
+    {
+      int t3 = threadIdx.x;
+      asm volatile("" : "+v"(t3));
+#pragma unroll
+      for (int c = 0; c < CPT; c++)
+        if (W % T == 0 || t3 + c * T < W) round_column(blk, t3 + c * T, c);
+    }
+    if (blk == 0) {
+      // park block 0's digits in its own words: thread t's column a, line r
+      // at element a R + r, read back by the same thread after the loop
+#pragma unroll
+      for (int c = 0; c < CPT; c++) {
+        const int a = t + c * T;
+        if (W % T == 0 || a < W) {
+#pragma unroll
+          for (int r = 0; r < R; r++) gstore(A.x + (size_t)a * R + r, lds[r * LP + lpad(a)]);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < CPT; c++)
+        if (W % T == 0 || t + c * T < W) carry_in_column(blk, t + c * T, c);
+      __syncthreads();
+      forward_store(blk);
+    }
+    if (next >= G) break;
+    blk = next;
+  }
+  if (first == 0) {
+    // block 0: its digits back (the words this thread parked), its rows'
+    // weights and flags, the last block's carries, then the forward half
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    __syncthreads();
+    row_weights(0, t);
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < CPT; c++) {
+      const int a = t + c * T;
+      if (W % T == 0 || a < W) {
+        uint64_t fl = 0;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          lds[r * LP + lpad(a)] = gload(A.x + (size_t)a * R + r);
+#pragma unroll
+          for (int e = 0; e < 2; e++) {
+            const int i = 2 * r + e;
+            const int f0 = fa_c[c] + fb[i];
+            const int wrap = f0 >= (int)A.n;
+            const int big = f0 - (wrap ? (int)A.n : 0) < (int)A.b;
+            fl |= ((uint64_t)wrap | ((uint64_t)big << 1)) << (2 * i);
+          }
+        }
+        flags[c] = fl;
+        carry_in_column(0, a, c);
+      }
+    }
+    __syncthreads();
+    forward_store(0);
+  }
+  const float m = block_max<T>((float)emax, red);
+  if (threadIdx.x == 0) {
+    atomicMax(&A.err[(epoch - 1) & A.err_mask], __float_as_uint(m));
+    // no carry workgroup reads `remaining`: the next square sweep does
+    if (first == 0 && A.remaining) *A.remaining -= 1;
+  }
+}
+
 // ------------------------------------------------------ load / unload (HBM)
 // load: balanced digits (natural order) -> weighted, line-transformed,
 // twiddled signal, the state the square sweep expects.
