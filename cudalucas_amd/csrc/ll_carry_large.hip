@@ -15,12 +15,19 @@ namespace llamd {
    carry_pass_table<W, R, T>() ? (const void *)line_pass_table<W, R, T> : nullptr,       \
    (const void *)carry_sweep_pipe<W, R, T>}
 
+// with a 512-thread form for schedules whose blocks fit in one round of it
+#define LL_CARRY_W(W, R)                                                                 \
+  {W, R, (const void *)carry_sweep<W, R, 256>, (const void *)load_lines<W, R, 256>,      \
+   (const void *)unload_lines<W, R, 256>, 256,                                           \
+   carry_pass_table<W, R, 256>() ? (const void *)line_pass_table<W, R, 256> : nullptr,   \
+   nullptr, carry_wide_form<W, R>()}
+
 extern const CarryKernel k_carry_large[] = {
-    LL_CARRY(256, 4),  LL_CARRY(512, 4),  LL_CARRY(1024, 4), LL_CARRY(2048, 2),
-    LL_CARRY(384, 4),  LL_CARRY(640, 4),  LL_CARRY(768, 4),  LL_CARRY(896, 4),
+    LL_CARRY_W(256, 4),  LL_CARRY_W(512, 4),  LL_CARRY_W(1024, 4), LL_CARRY(2048, 2),
+    LL_CARRY_W(384, 4),  LL_CARRY_W(640, 4),  LL_CARRY_W(768, 4),  LL_CARRY_W(896, 4),
     LL_CARRY_PIPE(1280, 4, 512), LL_CARRY_PIPE(1536, 4, 512), LL_CARRY_T(1792, 4, 512), LL_CARRY(1024, 2),
-    LL_CARRY(512, 8),  LL_CARRY_PIPE(1024, 8, 512), LL_CARRY(256, 8), LL_CARRY(384, 8),
-    LL_CARRY(640, 8),  LL_CARRY(768, 8), LL_CARRY_PIPE(896, 8, 512),
+    LL_CARRY_W(512, 8),  LL_CARRY_PIPE(1024, 8, 512), LL_CARRY_W(256, 8), LL_CARRY_W(384, 8),
+    LL_CARRY_W(640, 8),  LL_CARRY_W(768, 8), LL_CARRY_PIPE(896, 8, 512),
     LL_CARRY_T(512, 16, 512), LL_CARRY_PIPE(2048, 4, 512),
 };
 extern const size_t k_carry_large_count = sizeof(k_carry_large) / sizeof(k_carry_large[0]);

@@ -92,7 +92,7 @@ struct Schedule {
   const CarryKernel *ck;
   bool generic;   // the runtime-plan kernels (ll_generic.hip)
   uint32_t sq_grid;  // square-sweep workgroups along x (square_grid)
-  uint32_t ca_grid;  // carry-sweep workgroups along x (carry_grid)
+  CarryLaunch ca;     // carry-sweep kernel, workgroup size and grid (carry_launch)
 };
 
 bool seven_smooth(uint32_t x) {
@@ -458,9 +458,9 @@ static int build_sweeps(ll_ctx *c) {
   A.remaining = nullptr;
   A.stagger_sq = A.stagger_ca = 0;
   if (c->sch.sk->h) A.stagger_sq = sweep_stagger(c->sch.sk->fn, c->sch.sk->threads, 0, W / 2, 32 * H, c->device);
-  if (c->sch.ck->w) A.stagger_ca = sweep_stagger(c->sch.ck->carry, c->sch.ck->threads, 0, H / R, 16 * R * W, c->device);
   c->sch.sq_grid = square_grid(c->sch.sk, W, 1);
-  c->sch.ca_grid = carry_grid(c->sch.ck, H, R, 1);
+  c->sch.ca = carry_launch(c->sch.ck, H, R, 1);
+  if (c->sch.ck->w) A.stagger_ca = sweep_stagger(c->sch.ca.fn, c->sch.ca.threads, 0, H / R, 16 * R * W, c->device);
   if ((rc = dalloc(c, &c->d_args, 1))) return rc;
   if ((rc = dalloc(c, &c->d_args_unload, 1))) return rc;
   HIPCHK(c, hipMemcpyAsync(c->d_args, &A, sizeof(PlanArgs), hipMemcpyHostToDevice, c->stream));
@@ -545,7 +545,7 @@ static int build_graph(ll_ctx *c) {
   HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
   for (uint32_t i = 0; i < kGraphIters; i++) {
     (void)launch_square(c->sch.sk, c->d_args, 1, c->sch.W, c->sch.H, c->sch.sq_grid, c->stream);
-    (void)launch_carry(c->sch.ck, c->d_args, 1, c->sch.W, c->sch.H, c->sch.R, c->sch.ca_grid, c->stream);
+    (void)launch_carry(c->sch.ck, c->sch.ca, c->d_args, 1, c->sch.W, c->sch.H, c->sch.R, c->stream);
   }
   HIPCHK(c, hipStreamEndCapture(c->stream, &g));
   hipError_t e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
@@ -699,7 +699,7 @@ static int iterate_sweeps(ll_ctx *c, uint32_t count, const PlanArgs *dargs, cons
   }
   for (; left; left--) {
     HIPCHK(c, launch_square(c->sch.sk, dargs, 1, c->sch.W, c->sch.H, c->sch.sq_grid, c->stream));
-    HIPCHK(c, launch_carry(c->sch.ck, dargs, 1, c->sch.W, c->sch.H, c->sch.R, c->sch.ca_grid, c->stream));
+    HIPCHK(c, launch_carry(c->sch.ck, c->sch.ca, dargs, 1, c->sch.W, c->sch.H, c->sch.R, c->stream));
   }
   HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
   // the slots of epochs epoch .. epoch + count - 1, which may wrap the ring
@@ -840,7 +840,7 @@ int ll_profile_kernels(ll_ctx *c, uint32_t count, float *kernel_ms) {
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
     HIPCHK(c, launch_square(c->sch.sk, c->d_args, 1, c->sch.W, c->sch.H, c->sch.sq_grid, c->stream));
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-    HIPCHK(c, launch_carry(c->sch.ck, c->d_args, 1, c->sch.W, c->sch.H, c->sch.R, c->sch.ca_grid, c->stream));
+    HIPCHK(c, launch_carry(c->sch.ck, c->sch.ca, c->d_args, 1, c->sch.W, c->sch.H, c->sch.R, c->stream));
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
     HIPCHK(c, hipEventSynchronize(c->ev[2]));
     float a = 0.f, b = 0.f;
@@ -1035,8 +1035,8 @@ static int enqueue_sweeps(ll_batch *b, uint32_t iters) {
         const uint32_t k = (uint32_t)grp.members.size();
         const uint32_t grid = square_grid(grp.sch.sk, grp.sch.W, k);
         BCHK(b, launch_square(grp.sch.sk, grp.d_args, k, grp.sch.W, grp.sch.H, grid, b->stream));
-        const uint32_t cgrid = carry_grid(grp.sch.ck, grp.sch.H, grp.sch.R, k);
-        BCHK(b, launch_carry(grp.sch.ck, grp.d_args, k, grp.sch.W, grp.sch.H, grp.sch.R, cgrid, b->stream));
+        const CarryLaunch cl = carry_launch(grp.sch.ck, grp.sch.H, grp.sch.R, k);
+        BCHK(b, launch_carry(grp.sch.ck, cl, grp.d_args, k, grp.sch.W, grp.sch.H, grp.sch.R, b->stream));
       }
   }
   return LL_OK;

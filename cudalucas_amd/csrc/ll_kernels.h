@@ -107,6 +107,14 @@ struct CarryKernel {
   uint32_t threads;
   const void *table;  // builds the line pass table ptW (null: not used)
   const void *pipe;   // persistent form of the carry sweep (carry_sweep_pipe), or null
+  const void *wide;   // 512-thread form of a 256-thread sweep (same line table), or null
+};
+
+// The carry-sweep launch of a schedule: kernel, workgroup size and
+// workgroups along x (carry_launch).
+struct CarryLaunch {
+  const void *fn;
+  uint32_t threads, grid;
 };
 
 // The generic square / carry kernels (ll_generic.hip): one instantiation
@@ -178,14 +186,16 @@ uint32_t square_grid(const SquareKernel *k, uint32_t W, uint32_t tests);
 // grid: square_grid's value (the persistent form runs when grid < W / 2)
 hipError_t launch_square(const SquareKernel *k, const PlanArgs *d_args, uint32_t tests, uint32_t W,
                          uint32_t H, uint32_t grid, hipStream_t s);
-// Workgroups along x of a carry-sweep launch of `tests` tests with H / R
-// line blocks each on the current device: H / R, or one round of the
-// persistent form when k has one and the blocks would take more than one
-// round of the plain kernel.
-uint32_t carry_grid(const CarryKernel *k, uint32_t H, uint32_t R, uint32_t tests);
-// grid: carry_grid's value (the persistent form runs when grid < H / R)
-hipError_t launch_carry(const CarryKernel *k, const PlanArgs *d_args, uint32_t tests, uint32_t W,
-                        uint32_t H, uint32_t R, uint32_t grid, hipStream_t s);
+// The carry-sweep launch of `tests` tests with H / R line blocks each on
+// the current device: the persistent form (one round of workgroups) when
+// k has one and the blocks would take more than one round of the plain
+// kernel; else the 512-thread form when k has one, the blocks fit in one
+// round of it and outnumber half the CUs (twice the waves per CU where
+// there are about as many blocks as CUs); else the plain kernel, H / R
+// workgroups.
+CarryLaunch carry_launch(const CarryKernel *k, uint32_t H, uint32_t R, uint32_t tests);
+hipError_t launch_carry(const CarryKernel *k, const CarryLaunch &cl, const PlanArgs *d_args, uint32_t tests,
+                        uint32_t W, uint32_t H, uint32_t R, hipStream_t s);
 hipError_t launch_load(const CarryKernel *k, const PlanArgs *d_args, uint32_t tests, uint32_t W,
                        uint32_t H, uint32_t R, hipStream_t s);
 hipError_t launch_unload(const CarryKernel *k, const PlanArgs *d_args, uint32_t tests, uint32_t W,

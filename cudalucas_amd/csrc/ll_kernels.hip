@@ -290,8 +290,8 @@ uint32_t sweep_stagger(const void *fn, uint32_t threads, uint32_t dyn_lds, uint3
   return bytes_per_group / 6144;
 }
 
-// Workgroups of one round of a persistent sweep kernel on the current
-// device (its occupancy times the CUs), looked up once per kernel and device.
+// Workgroups of one round of a kernel on the current device (its occupancy
+// times the CUs; fn = null: the CUs), looked up once per kernel and device.
 static uint32_t one_round(const void *fn, uint32_t threads) {
   static std::mutex mu;
   static std::vector<std::tuple<const void *, int, uint32_t>> known;
@@ -300,8 +300,8 @@ static uint32_t one_round(const void *fn, uint32_t threads) {
   std::lock_guard<std::mutex> lock(mu);
   for (const auto &e : known)
     if (std::get<0>(e) == fn && std::get<1>(e) == dev) return std::get<2>(e);
-  int per_cu = 0, cus = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)threads, 0) != hipSuccess ||
+  int per_cu = 1, cus = 0;
+  if ((fn && hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)threads, 0) != hipSuccess) ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || per_cu <= 0 ||
       cus <= 0)
     return 0;
@@ -328,23 +328,40 @@ hipError_t launch_square(const SquareKernel *k, const PlanArgs *d_args, uint32_t
   return hipLaunchKernel(fn, dim3(grid, tests), dim3(k->threads), args, lds, s);
 }
 
-uint32_t carry_grid(const CarryKernel *k, uint32_t H, uint32_t R, uint32_t tests) {
+CarryLaunch carry_launch(const CarryKernel *k, uint32_t H, uint32_t R, uint32_t tests) {
   const uint32_t G = R ? H / R : 0;
-  if (!k->pipe || tests == 0) return G;
+  CarryLaunch c{k->carry, k->threads, G};
+  if (tests == 0) return c;
+  const uint64_t need = (uint64_t)G * tests;
   const uint32_t plain = one_round(k->carry, k->threads);
-  if (plain == 0 || (uint64_t)G * tests <= plain) return G;
-  const uint32_t r = one_round(k->pipe, k->threads) / tests;
-  return r > 0 && r < G ? r : G;
+  if (k->pipe && plain && need > plain) {
+    const uint32_t r = one_round(k->pipe, k->threads) / tests;
+    if (r > 0 && r < G) {
+      c.fn = k->pipe;
+      c.grid = r;
+    }
+    return c;
+  }
+  // (measured: with at most one block per two CUs the 256-thread form is
+  // as fast or faster, 512K words 0.0220 against 0.0228 ms per iteration)
+  if (k->wide) {
+    const uint32_t wr = one_round(k->wide, 512), cus = one_round(nullptr, 0);
+    if (wr && need <= wr && 2 * need > cus) {
+      c.fn = k->wide;
+      c.threads = 512;
+    }
+  }
+  return c;
 }
 
-hipError_t launch_carry(const CarryKernel *k, const PlanArgs *d_args, uint32_t tests, uint32_t W,
-                        uint32_t H, uint32_t R, uint32_t grid, hipStream_t s) {
+hipError_t launch_carry(const CarryKernel *k, const CarryLaunch &cl, const PlanArgs *d_args, uint32_t tests,
+                        uint32_t W, uint32_t H, uint32_t R, hipStream_t s) {
   const PlanArgs *a = d_args;
   void *args[] = {&a};
   const uint32_t lds = k->w ? 0 : carry_generic_lds(W, R);
-  if (grid == 0 || grid > H / R) return hipErrorInvalidValue;
-  const void *fn = k->pipe && grid < H / R ? k->pipe : k->carry;
-  return hipLaunchKernel(fn, dim3(grid, tests), dim3(k->threads), args, lds, s);
+  if (!cl.fn || cl.grid == 0 || cl.grid > H / R || (cl.fn != k->carry && cl.fn != k->pipe && cl.fn != k->wide))
+    return hipErrorInvalidValue;
+  return hipLaunchKernel(cl.fn, dim3(cl.grid, tests), dim3(cl.threads), args, lds, s);
 }
 
 hipError_t launch_load(const CarryKernel *k, const PlanArgs *d_args, uint32_t tests, uint32_t W,

@@ -856,6 +856,18 @@ __global__ __launch_bounds__(T, (T >= 512 ? 1 : carry_groups_per_cu<W, R, T>()))
   }
 }
 
+// The 512-thread form of carry_sweep<W, R, 256> (more waves per CU where a
+// schedule has no more line blocks than one round of it holds).  It reads
+// the same line table, or none, so a context built for one form can launch
+// either.
+template <int W, int R>
+const void *carry_wide_form() {
+  static_assert(carry_pass_table<W, R, 256>() == carry_pass_table<W, R, 512>() &&
+                    (!carry_pass_table<W, R, 256>() || auto_plan_uniform<W, R, 256>() == auto_plan_uniform<W, R, 512>()),
+                "the 512-thread carry sweep must read the 256-thread form's line table");
+  return (const void *)carry_sweep<W, R, 512>;
+}
+
 // ------------------------------------------------- persistent carry sweep
 // For lengths whose H / R line blocks take more than one round of
 // carry_sweep at T >= 512 (one workgroup per CU): one round of workgroups,
