@@ -509,6 +509,10 @@ static int build_resident(ll_ctx *c) {
   if ((rc = upload(c, (int32_t **)&D.fT, fT))) return rc;
   if ((rc = upload(c, (int32_t **)&D.fC, fC))) return rc;
   if ((rc = upload(c, (double **)&D.qc, qc))) return rc;
+  cplx *pt = nullptr;
+  if ((rc = dalloc(c, &pt, M))) return rc;
+  HIPCHK(c, launch_resident_table(c->sch.rk, D.qc, pt, c->stream));
+  D.pt = pt;
   if ((rc = dalloc(c, &c->r_err, c->ring_size))) return rc;
   if ((rc = dalloc(c, &c->r_offset, 1))) return rc;
   if ((rc = dalloc(c, &c->d_rdesc, 1))) return rc;

@@ -86,6 +86,7 @@ struct ResidentDesc {
   const double *wT, *iwTs, *wC, *iwC;
   const int32_t *fT, *fC;
   const double *qc;   // cos(2 pi e/M), e <= M/4
+  const cplx *pt;     // the M-point plan's per-pass twiddles, M - 1 roots
   float *err;         // iters floats
   uint32_t *offset;   // shift offset, read at start and written at the end
   uint32_t *remaining; // iterations left (batched tests), or null
@@ -147,6 +148,7 @@ struct ResidentKernel {
   uint32_t m;
   const void *fn;
   uint32_t threads;
+  const void *table;  // builds the per-pass twiddles of the M-point plan (ResidentDesc::pt)
 };
 
 // The no-allocation iteration path (ll_iterate_noalloc) keeps the carry
@@ -212,5 +214,6 @@ hipError_t launch_copy(const cplx *in, cplx *out, size_t n, hipStream_t s);
 hipError_t launch_square_table(const SquareKernel *k, const double *qcH, cplx *ptH, hipStream_t s);
 // ptW of a carry kernel's line plan from qcW (k->table must be set)
 hipError_t launch_carry_table(const CarryKernel *k, const double *qcW, cplx *ptW, hipStream_t s);
+hipError_t launch_resident_table(const ResidentKernel *k, const double *qc, cplx *pt, hipStream_t s);
 
 }  // namespace llamd

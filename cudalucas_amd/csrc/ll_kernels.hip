@@ -58,10 +58,19 @@ __global__ __launch_bounds__(T) void resident_kernel(const ResidentDesc *descs) 
   const uint32_t p = D.p, b = p % N, q = p / N;
   const uint32_t j0 = (uint32_t)tid * C;
   uint32_t offset = *D.offset;
+  // Workgroups of at most 256 threads (one wave per SIMD, registers to
+  // spare) read each thread's Stockham twiddles once from the plan's
+  // per-pass table and keep them in registers for every transform of the
+  // launch (the inverse uses their conjugates); 512-thread workgroups take
+  // them from the quarter-wave table in LDS, which costs them no registers.
+  constexpr bool CT = T <= 256;
+  PassTwiddles<M, 1, T> ptw;
+  if constexpr (CT) ptw.init_table(D.pt);
   __syncthreads();
   for (uint32_t it = 0; it < iters; it++) {
     if (tid == 0) s_m2 = minus2_position(offset, p, N);
-    fft_lds<M, 1, T, false>(lds, tw);
+    if constexpr (CT) fft_lds_cached<M, 1, T, false>(lds, ptw);
+    else fft_lds<M, 1, T, false>(lds, tw);
     for (int k = tid; k <= M / 2; k += T) {
       const int kb = (M - k) % M;
       cplx a = lds[lpad(k)], bb = lds[lpad(kb)];
@@ -70,7 +79,8 @@ __global__ __launch_bounds__(T) void resident_kernel(const ResidentDesc *descs) 
       if (kb != k) lds[lpad(kb)] = bb;
     }
     __syncthreads();
-    fft_lds<M, 1, T, true>(lds, tw);
+    if constexpr (CT) fft_lds_cached<M, 1, T, true>(lds, ptw);
+    else fft_lds<M, 1, T, true>(lds, tw);
     const Minus2 m2 = s_m2;
     // bit i: word i's weight exponent sum wrapped / word i is a big digit
     uint64_t wraps = 0, bigs = 0;
@@ -217,7 +227,7 @@ hipError_t noalloc_scratch(int device, void **ptr) {
 #define LL_SQUARE_PIPE(H, T)                                                                           \
   {H, (const void *)square_sweep<H, T>, T, (const void *)column_pass_table<H>,                         \
    (const void *)square_sweep_pipe<H, T>}
-#define LL_RESIDENT(M, T) {M, (const void *)resident_kernel<M, T>, T}
+#define LL_RESIDENT(M, T) {M, (const void *)resident_kernel<M, T>, T, (const void *)column_pass_table<M>}
 
 static const SquareKernel k_square[] = {
     LL_SQUARE_T(128, 64), LL_SQUARE(256), LL_SQUARE(512), LL_SQUARE(1024), LL_SQUARE(2048), LL_SQUARE_PIPE(4096, 512),
@@ -239,9 +249,9 @@ extern const size_t k_carry_huge1_count;
 
 static const ResidentKernel k_resident[] = {
     LL_RESIDENT(256, 64),   LL_RESIDENT(512, 128),  LL_RESIDENT(1024, 256),
-    LL_RESIDENT(2048, 256), LL_RESIDENT(4096, 512), LL_RESIDENT(8192, 512),
+    LL_RESIDENT(2048, 256), LL_RESIDENT(4096, 256), LL_RESIDENT(8192, 512),
     LL_RESIDENT(1536, 256), LL_RESIDENT(2560, 256), LL_RESIDENT(3584, 256),
-    LL_RESIDENT(3072, 512), LL_RESIDENT(5120, 512), LL_RESIDENT(6144, 512),
+    LL_RESIDENT(3072, 256), LL_RESIDENT(5120, 512), LL_RESIDENT(6144, 512),
     LL_RESIDENT(7168, 512),
 };
 
@@ -401,6 +411,11 @@ hipError_t launch_resident_load(const ResidentDesc *d_descs, uint32_t count, con
 
 hipError_t launch_square_table(const SquareKernel *k, const double *qcH, cplx *ptH, hipStream_t s) {
   void *args[] = {&qcH, &ptH};
+  return hipLaunchKernel(k->table, dim3(1), dim3(256), args, 0, s);
+}
+
+hipError_t launch_resident_table(const ResidentKernel *k, const double *qc, cplx *pt, hipStream_t s) {
+  void *args[] = {&qc, &pt};
   return hipLaunchKernel(k->table, dim3(1), dim3(256), args, 0, s);
 }
 

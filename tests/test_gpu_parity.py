@@ -62,6 +62,23 @@ def test_resident_vs_gmp(p, n, iters, offset):
     assert m < ROUNDOFF_LIMIT
 
 
+# every LDS-resident instantiation (M = n / 2 from 256 to 8192 points, the
+# 7-smooth ones included) at about 16.5 bits/word
+RESIDENT_LENGTHS = [512, 1024, 2048, 3072, 4096, 5120, 6144, 7168, 8192, 10240, 12288, 14336, 16384]
+
+
+@pytest.mark.parametrize("n", RESIDENT_LENGTHS)
+def test_resident_lengths_vs_gmp(n):
+    p = W.next_prime(int(n * 16.5))
+    announce("resident length p=%d n=%d" % (p, n))
+    with L.LLContext(p, n) as c:
+        assert c.info()["path"] == "resident"
+    w, r, m = L.run_ll(p, n, 300, 7)
+    g = O.gmp_ll(p, 300)
+    assert r == O.res64(g)
+    assert m < ROUNDOFF_LIMIT
+
+
 def test_small_exponent_full_run_vs_gmp():
     p, n = W.SMALL_P, W.SMALL_N
     announce("small exponent full run p=%d n=%d" % (p, n))
