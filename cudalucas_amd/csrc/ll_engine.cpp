@@ -130,8 +130,8 @@ std::vector<uint32_t> generic_radices(uint32_t L) {
 // lines per carry block whose LDS fits; o may force W, H or R.  Preferred:
 // H near 2048, R = 8 (128-byte column runs) or 4, fewer line passes.  With
 // max_lds below kGenMaxLds (the batched engine: several workgroups per CU)
-// both sweeps' LDS stay under it and W near H / 2 is preferred, which
-// balances the two sweeps' workgroup counts.
+// both sweeps' LDS stay under it and the split with the fewest workgroups
+// (largest items) is preferred.
 bool pick_generic(uint32_t M, uint32_t want_r, const ll_options *o, Schedule *s,
                   uint32_t max_lds = kGenMaxLds) {
   double best = 1e30;
@@ -147,8 +147,20 @@ bool pick_generic(uint32_t M, uint32_t want_r, const ll_options *o, Schedule *s,
     for (uint32_t R = 16; R >= 1; R /= 2) {
       if (H % R || carry_generic_lds(W, R) > max_lds) continue;
       if (want_r && want_r != R) continue;
-      double cost = balanced ? fabs(log2((double)H / W) - 1.0) * 2.0 : fabs(log2((double)H) - 11.0) * 2.0;
-      cost += R == 8 ? 0.0 : R == 4 ? 0.5 : R == 16 ? 0.7 : R == 2 ? 1.5 : 4.0;
+      // Under an LDS bound (the batched engine's work items): the fewest
+      // items, W / 2 column pairs plus H / R line blocks, so each item has
+      // the most work for its fixed latency (measured on the 256-exponent
+      // workload: 0.624 ms per batch iteration at H near 2W and R = 8,
+      // 0.511 ms with the fewest items); R below 8 reads columns in short
+      // runs.  Otherwise columns near 2048 points and R = 8.
+      double cost;
+      if (balanced) {
+        cost = log2((double)(W / 2 + H / R)) * 2.0;
+        cost += R >= 8 ? 0.0 : R == 4 ? 0.6 : R == 2 ? 1.5 : 4.0;
+      } else {
+        cost = fabs(log2((double)H) - 11.0) * 2.0;
+        cost += R == 8 ? 0.0 : R == 4 ? 0.5 : R == 16 ? 0.7 : R == 2 ? 1.5 : 4.0;
+      }
       cost += 0.1 * (double)generic_radices(W).size();
       if (cost < best) {
         best = cost;
