@@ -58,12 +58,12 @@ __global__ __launch_bounds__(T) void resident_kernel(const ResidentDesc *descs) 
   const uint32_t p = D.p, b = p % N, q = p / N;
   const uint32_t j0 = (uint32_t)tid * C;
   uint32_t offset = *D.offset;
-  // Workgroups of at most 256 threads (one wave per SIMD, registers to
-  // spare) read each thread's Stockham twiddles once from the plan's
-  // per-pass table and keep them in registers for every transform of the
-  // launch (the inverse uses their conjugates); 512-thread workgroups take
-  // them from the quarter-wave table in LDS, which costs them no registers.
-  constexpr bool CT = T <= 256;
+  // Up to 4096 points (registers to spare) each thread reads its Stockham
+  // twiddles once from the plan's per-pass table and keeps them in
+  // registers for every transform of the launch (the inverse uses their
+  // conjugates); longer transforms take them from the quarter-wave table
+  // in LDS, which costs them no registers.
+  constexpr bool CT = T <= 256 || M <= 4096;
   PassTwiddles<M, 1, T> ptw;
   if constexpr (CT) ptw.init_table(D.pt);
   __syncthreads();
@@ -248,9 +248,9 @@ extern const CarryKernel k_carry_huge1[];
 extern const size_t k_carry_huge1_count;
 
 static const ResidentKernel k_resident[] = {
-    LL_RESIDENT(256, 64),   LL_RESIDENT(512, 128),  LL_RESIDENT(1024, 256),
-    LL_RESIDENT(2048, 256), LL_RESIDENT(4096, 256), LL_RESIDENT(8192, 512),
-    LL_RESIDENT(1536, 256), LL_RESIDENT(2560, 256), LL_RESIDENT(3584, 256),
+    LL_RESIDENT(256, 64),   LL_RESIDENT(512, 128),  LL_RESIDENT(1024, 512),
+    LL_RESIDENT(2048, 512), LL_RESIDENT(4096, 512), LL_RESIDENT(8192, 512),
+    LL_RESIDENT(1536, 512), LL_RESIDENT(2560, 256), LL_RESIDENT(3584, 256),
     LL_RESIDENT(3072, 256), LL_RESIDENT(5120, 512), LL_RESIDENT(6144, 512),
     LL_RESIDENT(7168, 512),
 };
