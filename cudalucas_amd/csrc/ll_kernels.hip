@@ -345,9 +345,11 @@ CarryLaunch carry_launch(const CarryKernel *k, uint32_t H, uint32_t R, uint32_t 
   const uint64_t need = (uint64_t)G * tests;
   const uint32_t plain = one_round(k->carry, k->threads);
   if (k->pipe && plain && need > plain) {
-    const uint32_t r = one_round(k->pipe, k->threads) / tests;
+    // (the persistent forms run 512 threads whatever the plain kernel's)
+    const uint32_t r = one_round(k->pipe, 512) / tests;
     if (r > 0 && r < G) {
       c.fn = k->pipe;
+      c.threads = 512;
       c.grid = r;
     }
     return c;

@@ -1175,6 +1175,16 @@ __global__ __launch_bounds__(T, 1) void carry_sweep_pipe(const PlanArgs *__restr
   }
 }
 
+// The persistent form (512 threads) of carry_sweep<W, R, 256>, for
+// schedules whose line blocks take several rounds; same line table.
+template <int W, int R>
+const void *carry_pipe_form() {
+  static_assert(carry_pass_table<W, R, 256>() == carry_pass_table<W, R, 512>() &&
+                    (!carry_pass_table<W, R, 256>() || auto_plan_uniform<W, R, 256>() == auto_plan_uniform<W, R, 512>()),
+                "the persistent carry sweep must read the 256-thread form's line table");
+  return (const void *)carry_sweep_pipe<W, R, 512>;
+}
+
 // ------------------------------------------------------ load / unload (HBM)
 // load: balanced digits (natural order) -> weighted, line-transformed,
 // twiddled signal, the state the square sweep expects.
