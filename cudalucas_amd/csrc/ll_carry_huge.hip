@@ -1,22 +1,9 @@
 // Carry-sweep / load / unload instantiations: lines of 2268..4096 points
 // (R = 2) beside 4096- and 2048-point columns, for the 10240K..32768K
 // lengths of the reference's list whose line has a template.
-#include "ll_sweeps.h"
+#include "ll_carry_table.h"
 
 namespace llamd {
-
-#define LL_CARRY_T(W, R, T)                                                              \
-  {W, R, (const void *)carry_sweep<W, R, T>, (const void *)load_lines<W, R, T>,          \
-   (const void *)unload_lines<W, R, T>, T,                                               \
-   carry_pass_table<W, R, T>() ? (const void *)line_pass_table<W, R, T> : nullptr}
-
-// with the persistent 512-thread form for schedules whose blocks take
-// several rounds
-#define LL_CARRY_HP(W, R)                                                                \
-  {W, R, (const void *)carry_sweep<W, R, 256>, (const void *)load_lines<W, R, 256>,      \
-   (const void *)unload_lines<W, R, 256>, 256,                                           \
-   carry_pass_table<W, R, 256>() ? (const void *)line_pass_table<W, R, 256> : nullptr,   \
-   carry_pipe_form<W, R>(), nullptr}
 
 extern const CarryKernel k_carry_huge[] = {
     LL_CARRY_T(2268, 2, 256), LL_CARRY_HP(2500, 2), LL_CARRY_HP(2744, 2),

@@ -1,26 +1,7 @@
 // Carry-sweep / load / unload instantiations: line lengths 256..2048.
-#include "ll_sweeps.h"
+#include "ll_carry_table.h"
 
 namespace llamd {
-
-#define LL_CARRY_T(W, R, T)                                                              \
-  {W, R, (const void *)carry_sweep<W, R, T>, (const void *)load_lines<W, R, T>,          \
-   (const void *)unload_lines<W, R, T>, T,                                               \
-   carry_pass_table<W, R, T>() ? (const void *)line_pass_table<W, R, T> : nullptr}
-#define LL_CARRY(W, R) LL_CARRY_T(W, R, 256)
-// with the persistent form for lengths whose blocks take several rounds
-#define LL_CARRY_PIPE(W, R, T)                                                           \
-  {W, R, (const void *)carry_sweep<W, R, T>, (const void *)load_lines<W, R, T>,          \
-   (const void *)unload_lines<W, R, T>, T,                                               \
-   carry_pass_table<W, R, T>() ? (const void *)line_pass_table<W, R, T> : nullptr,       \
-   (const void *)carry_sweep_pipe<W, R, T>}
-
-// with a 512-thread form for schedules whose blocks fit in one round of it
-#define LL_CARRY_W(W, R)                                                                 \
-  {W, R, (const void *)carry_sweep<W, R, 256>, (const void *)load_lines<W, R, 256>,      \
-   (const void *)unload_lines<W, R, 256>, 256,                                           \
-   carry_pass_table<W, R, 256>() ? (const void *)line_pass_table<W, R, 256> : nullptr,   \
-   nullptr, carry_wide_form<W, R>()}
 
 extern const CarryKernel k_carry_large[] = {
     LL_CARRY_W(256, 4),  LL_CARRY_W(512, 4),  LL_CARRY_W(1024, 4), LL_CARRY(2048, 2),

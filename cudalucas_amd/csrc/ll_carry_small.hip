@@ -1,20 +1,7 @@
 // Carry-sweep / load / unload instantiations: short lines (72..448) for 18K..224K-word FFTs.
-#include "ll_sweeps.h"
+#include "ll_carry_table.h"
 
 namespace llamd {
-
-#define LL_CARRY_T(W, R, T)                                                              \
-  {W, R, (const void *)carry_sweep<W, R, T>, (const void *)load_lines<W, R, T>,          \
-   (const void *)unload_lines<W, R, T>, T,                                               \
-   carry_pass_table<W, R, T>() ? (const void *)line_pass_table<W, R, T> : nullptr}
-#define LL_CARRY(W, R) LL_CARRY_T(W, R, 256)
-
-// with a 512-thread form for schedules whose blocks fit in one round of it
-#define LL_CARRY_W(W, R)                                                                 \
-  {W, R, (const void *)carry_sweep<W, R, 256>, (const void *)load_lines<W, R, 256>,      \
-   (const void *)unload_lines<W, R, 256>, 256,                                           \
-   carry_pass_table<W, R, 256>() ? (const void *)line_pass_table<W, R, 256> : nullptr,   \
-   nullptr, carry_wide_form<W, R>()}
 
 extern const CarryKernel k_carry_small[] = {
     LL_CARRY_T(72, 8, 64),   LL_CARRY_T(80, 8, 64),   LL_CARRY_T(128, 8, 64),

@@ -1,20 +1,8 @@
 // Carry-sweep / load / unload instantiations: lines with odd factors, 1080..2880 points (R = 4 and 2), for the
 // 4320K..11520K-word lengths of the reference's list that have 2048-point columns.
-#include "ll_sweeps.h"
+#include "ll_carry_table.h"
 
 namespace llamd {
-
-#define LL_CARRY_T(W, R, T)                                                              \
-  {W, R, (const void *)carry_sweep<W, R, T>, (const void *)load_lines<W, R, T>,          \
-   (const void *)unload_lines<W, R, T>, T,                                               \
-   carry_pass_table<W, R, T>() ? (const void *)line_pass_table<W, R, T> : nullptr}
-
-// with the persistent form for lengths whose blocks take several rounds
-#define LL_CARRY_PIPE(W, R, T)                                                           \
-  {W, R, (const void *)carry_sweep<W, R, T>, (const void *)load_lines<W, R, T>,          \
-   (const void *)unload_lines<W, R, T>, T,                                               \
-   carry_pass_table<W, R, T>() ? (const void *)line_pass_table<W, R, T> : nullptr,       \
-   (const void *)carry_sweep_pipe<W, R, T>}
 
 extern const CarryKernel k_carry_wide[] = {
     LL_CARRY_PIPE(1080, 4, 512), LL_CARRY_PIPE(1152, 4, 512), LL_CARRY_PIPE(1176, 4, 512),
