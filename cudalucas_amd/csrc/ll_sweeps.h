@@ -60,6 +60,21 @@ __device__ __forceinline__ void stagger_start(uint32_t sixteenths) {
   }
 }
 
+// Column pair of square-sweep workgroup i of n.  With lines tiled by R < 8
+// a column is read in runs of 16 R bytes, so S = 8 / R neighbouring columns
+// share each 128-byte line; workgroups i, i + 8, ..., i + 8 (S - 1), which
+// the dispatcher sends one after another to the same XCD, take S
+// neighbouring pairs, so that a line's parts meet in that XCD's L2 instead
+// of each being fetched from HBM.  A bijection of [0, n); the identity for
+// R = 8 and for the last n mod 8S workgroups.
+__device__ __forceinline__ uint32_t pair_of_workgroup(uint32_t i, uint32_t n, uint32_t rshift) {
+  if (rshift >= 3) return i;
+  const uint32_t S = 8u >> rshift, span = 8 * S;
+  if (i >= n / span * span) return i;
+  const uint32_t x = i % 8, q = i / 8;
+  return span * (q / S) + S * x + q % S;
+}
+
 __device__ __forceinline__ cplx twiddle_m(const TwoLevel &t, uint32_t e) {
   const uint32_t k = 2 * (e >> t.shift);
   const cplx h = gload(t.hi + k), hr = gload(t.hi + k + 1);
@@ -208,7 +223,7 @@ __global__ __launch_bounds__(T, 2) void square_sweep(const PlanArgs *__restrict_
   __shared__ cplx lds[2 * LP];
   const int tid = threadIdx.x;
   const uint32_t W = A.W;
-  const uint32_t g = blockIdx.x;
+  const uint32_t g = pair_of_workgroup(blockIdx.x, gridDim.x, A.rshift);
   const uint32_t c0 = g;
   const uint32_t c1 = g == 0 ? W / 2 : W - g;
   // column c, element b lives at ((b >> rs) W + c) << rs | (b & rmask)
@@ -403,7 +418,7 @@ __global__ __launch_bounds__(T, 1) void square_sweep_pipe(const PlanArgs *__rest
   const int tid = threadIdx.x;
   const uint32_t W = A.W, npairs = W / 2;
   const uint32_t rs = A.rshift, rmask = (1u << rs) - 1;
-  uint32_t g = blockIdx.x;
+  uint32_t g = pair_of_workgroup(blockIdx.x, gridDim.x, A.rshift);
   if (g >= npairs) return;
   auto partner = [&](uint32_t gg) { return gg == 0 ? W / 2 : W - gg; };
   cplx n0[NE], n1[NE];  // the pair being loaded
