@@ -320,8 +320,10 @@ __device__ __forceinline__ void square_generic_body(const PlanArgs &A, const uin
 __global__ __launch_bounds__(kGenThreads, kGenWavesPerEU) void square_generic(const PlanArgs *__restrict__ args) {
   extern __shared__ cplx dyn[];
   const PlanArgs &A = args[blockIdx.y];
-  if (A.H <= (uint32_t)kStage) square_generic_body<true>(A, blockIdx.x, dyn);
-  else square_generic_body<false>(A, blockIdx.x, dyn);
+  // neighbouring column pairs on one XCD, as square_sweep
+  const uint32_t g = pair_of_workgroup(blockIdx.x, gridDim.x, A.rshift);
+  if (A.H <= (uint32_t)kStage) square_generic_body<true>(A, g, dyn);
+  else square_generic_body<false>(A, g, dyn);
 }
 
 // ---------------------------------------------------------- carry sweep
