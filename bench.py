@@ -6,7 +6,8 @@ length N = 4,194,304 words (18.36 bits/word), s_0 = 4.  One step is one LL
 iteration (square sweep + carry sweep, hipGraph-replayed).  Prints one JSON
 line: ms per iteration over exactly --steps iterations after --warmup
 untimed ones, the roofline of the dominant kernel (algorithmic bytes
-16 N per sweep launch over its hipEvent-measured average duration), and a
+16 N per sweep launch over its average duration, from events set at each
+profiled kernel's own start and end), and a
 CPU baseline: the oracle's IBDWT restatement of the same (p, N) over a
 bounded sample, on every host core the process may use (OMP_NUM_THREADS,
 else the affinity mask; "cores" in the line) and on one core ("one_core"),

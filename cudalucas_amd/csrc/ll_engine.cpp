@@ -853,15 +853,16 @@ int ll_profile_kernels(ll_ctx *c, uint32_t count, float *kernel_ms) {
   double sq = 0.0, ca = 0.0;
   GpuLock lock(c->device);
   for (uint32_t i = 0; i < count; i++) {
-    HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-    HIPCHK(c, launch_square(c->sch.sk, c->d_args, 1, c->sch.W, c->sch.H, c->sch.sq_grid, c->stream));
-    HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-    HIPCHK(c, launch_carry(c->sch.ck, c->sch.ca, c->d_args, 1, c->sch.W, c->sch.H, c->sch.R, c->stream));
-    HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
-    HIPCHK(c, hipEventSynchronize(c->ev[2]));
+    // the events take each kernel's own start and end (hipExtLaunchKernel),
+    // not the dispatch gaps between the launches
+    HIPCHK(c, launch_square(c->sch.sk, c->d_args, 1, c->sch.W, c->sch.H, c->sch.sq_grid, c->stream, c->ev[0],
+                            c->ev[1]));
+    HIPCHK(c, launch_carry(c->sch.ck, c->sch.ca, c->d_args, 1, c->sch.W, c->sch.H, c->sch.R, c->stream, c->ev[2],
+                           c->ev[3]));
+    HIPCHK(c, hipEventSynchronize(c->ev[3]));
     float a = 0.f, b = 0.f;
     HIPCHK(c, hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
-    HIPCHK(c, hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
+    HIPCHK(c, hipEventElapsedTime(&b, c->ev[2], c->ev[3]));
     sq += a;
     ca += b;
   }

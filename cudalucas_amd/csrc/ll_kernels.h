@@ -185,9 +185,13 @@ uint32_t sweep_stagger(const void *fn, uint32_t threads, uint32_t dyn_lds, uint3
 // of the persistent form when k has one and W / 2 x tests pairs would take
 // more than one round of the plain kernel.
 uint32_t square_grid(const SquareKernel *k, uint32_t W, uint32_t tests);
-// grid: square_grid's value (the persistent form runs when grid < W / 2)
+// grid: square_grid's value (the persistent form runs when grid < W / 2).
+// start / stop (launch_square, launch_carry): if set, events that take the
+// kernel's own start and end times (hipExtLaunchKernel), for timing one
+// launch without the dispatch gaps around it.
 hipError_t launch_square(const SquareKernel *k, const PlanArgs *d_args, uint32_t tests, uint32_t W,
-                         uint32_t H, uint32_t grid, hipStream_t s);
+                         uint32_t H, uint32_t grid, hipStream_t s, hipEvent_t start = nullptr,
+                         hipEvent_t stop = nullptr);
 // The carry-sweep launch of `tests` tests with H / R line blocks each on
 // the current device: the persistent form (one round of workgroups) when
 // k has one and the blocks would take more than one round of the plain
@@ -197,7 +201,8 @@ hipError_t launch_square(const SquareKernel *k, const PlanArgs *d_args, uint32_t
 // workgroups.
 CarryLaunch carry_launch(const CarryKernel *k, uint32_t H, uint32_t R, uint32_t tests);
 hipError_t launch_carry(const CarryKernel *k, const CarryLaunch &cl, const PlanArgs *d_args, uint32_t tests,
-                        uint32_t W, uint32_t H, uint32_t R, hipStream_t s);
+                        uint32_t W, uint32_t H, uint32_t R, hipStream_t s, hipEvent_t start = nullptr,
+                        hipEvent_t stop = nullptr);
 hipError_t launch_load(const CarryKernel *k, const PlanArgs *d_args, uint32_t tests, uint32_t W,
                        uint32_t H, uint32_t R, hipStream_t s);
 hipError_t launch_unload(const CarryKernel *k, const PlanArgs *d_args, uint32_t tests, uint32_t W,

@@ -10,6 +10,7 @@
 //  * two sweeps over HBM for large N (ll_sweeps.h): square_sweep over
 //    column pairs, carry_sweep over blocks of lines, carries handed between
 //    blocks through an epoch-tagged shuttle.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -328,14 +329,22 @@ uint32_t square_grid(const SquareKernel *k, uint32_t W, uint32_t tests) {
   return r > 0 && r < W / 2 ? r : W / 2;
 }
 
+// A launch whose start and stop events (if set) take the kernel's own
+// execution times.
+static hipError_t launch_timed(const void *fn, dim3 grid, dim3 block, void **args, size_t lds, hipStream_t s,
+                               hipEvent_t start, hipEvent_t stop) {
+  if (start || stop) return hipExtLaunchKernel(fn, grid, block, args, lds, s, start, stop, 0);
+  return hipLaunchKernel(fn, grid, block, args, lds, s);
+}
+
 hipError_t launch_square(const SquareKernel *k, const PlanArgs *d_args, uint32_t tests, uint32_t W,
-                         uint32_t H, uint32_t grid, hipStream_t s) {
+                         uint32_t H, uint32_t grid, hipStream_t s, hipEvent_t start, hipEvent_t stop) {
   const PlanArgs *a = d_args;
   void *args[] = {&a};
   const uint32_t lds = k->h ? 0 : square_generic_lds(H);
   if (grid == 0 || grid > W / 2) return hipErrorInvalidValue;
   const void *fn = k->pipe && grid < W / 2 ? k->pipe : k->fn;
-  return hipLaunchKernel(fn, dim3(grid, tests), dim3(k->threads), args, lds, s);
+  return launch_timed(fn, dim3(grid, tests), dim3(k->threads), args, lds, s, start, stop);
 }
 
 CarryLaunch carry_launch(const CarryKernel *k, uint32_t H, uint32_t R, uint32_t tests) {
@@ -367,13 +376,13 @@ CarryLaunch carry_launch(const CarryKernel *k, uint32_t H, uint32_t R, uint32_t 
 }
 
 hipError_t launch_carry(const CarryKernel *k, const CarryLaunch &cl, const PlanArgs *d_args, uint32_t tests,
-                        uint32_t W, uint32_t H, uint32_t R, hipStream_t s) {
+                        uint32_t W, uint32_t H, uint32_t R, hipStream_t s, hipEvent_t start, hipEvent_t stop) {
   const PlanArgs *a = d_args;
   void *args[] = {&a};
   const uint32_t lds = k->w ? 0 : carry_generic_lds(W, R);
   if (!cl.fn || cl.grid == 0 || cl.grid > H / R || (cl.fn != k->carry && cl.fn != k->pipe && cl.fn != k->wide))
     return hipErrorInvalidValue;
-  return hipLaunchKernel(cl.fn, dim3(cl.grid, tests), dim3(cl.threads), args, lds, s);
+  return launch_timed(cl.fn, dim3(cl.grid, tests), dim3(cl.threads), args, lds, s, start, stop);
 }
 
 hipError_t launch_load(const CarryKernel *k, const PlanArgs *d_args, uint32_t tests, uint32_t W,

@@ -123,8 +123,10 @@ int ll_get_info(const ll_ctx *ctx, ll_info *info);
 double ll_last_iterate_ms(const ll_ctx *ctx);
 
 /* Per-kernel timing, the analog of kernel_percentages
- * (CUDALucas.cu:2002-2140): runs `count` iterations launching each kernel
- * between its own pair of hipEvents and returns the average duration in ms:
+ * (CUDALucas.cu:2002-2140): runs `count` iterations, each kernel launched
+ * with a pair of hipEvents that take its own start and end
+ * (hipExtLaunchKernel, so the dispatch gaps between launches are not
+ * counted), and returns the average duration in ms:
  * kernel_ms[0] = square sweep (or one whole iteration of the LDS-resident
  * kernel), kernel_ms[1] = carry sweep (0 for the resident path).  Per-
  * iteration errors are not collected. */
